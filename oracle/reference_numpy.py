@@ -1,0 +1,770 @@
+"""NumPy/SciPy FP64 restatement of the reference filamentation solver.
+
+TEST INFRASTRUCTURE ONLY -- the oracle. Imported by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``;
+never by the product package.
+
+Every function cites the reference file:line it restates. ``D3`` below is
+``.github/deliverable (3)``. cupy calls become their NumPy/SciPy twins:
+``cp.dot`` -> ``np.dot`` (complex128 ZGEMM after upcast, as cupy does),
+``cp.fft`` -> ``numpy.fft`` (same conventions), cupyx ``j0/j1`` -> scipy,
+cupyx ``PchipInterpolator`` -> scipy's (same algorithm), and the NVRTC rate
+kernel -> a time loop vectorised over the radial rows.
+
+Extensions that the benchmark workloads need and that the reference lacks are
+marked EXTENSION and default to the reference behaviour:
+
+* ``Config.t_window``: full width of the time window (reference: fixed 10*tp,
+  ``D3/sim/grids.py:32-35``);
+* ``Config.write_output``: skip the npz/json writes (tests);
+* ``envelope_lens_noise``: collimated Gaussian with a thin-lens phase and
+  seeded complex noise (the workload inputs).
+
+Parity status per function (what pins it) is listed in ``PINS`` at the bottom.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import datetime
+from dataclasses import dataclass
+from typing import Callable, Optional, Union
+
+import numpy as np
+from scipy.constants import c, epsilon_0, m_e
+from scipy.constants import elementary_charge as q_e
+from scipy.interpolate import PchipInterpolator
+from scipy.special import dawsn, ellipe, ellipk, j0, j1, jn_zeros
+
+HBAR = 1.054571817e-34
+
+# --------------------------------------------------------------------------
+# Sellmeier (D3/sim/keldysh.py:22-29)
+# --------------------------------------------------------------------------
+SELLMEIER_B = np.array([0.6961663, 0.4079426, 0.8974794])
+SELLMEIER_L2 = np.array([0.0684043, 0.1162414, 9.896161]) ** 2
+
+
+def n_sellmeier(lam_m):
+    """Three-term Malitson index of fused silica (keldysh.py:25-29)."""
+    lu2 = (lam_m * 1e6) ** 2
+    acc = 0.0
+    for B, L2 in zip(SELLMEIER_B, SELLMEIER_L2):
+        acc = acc + B * lu2 / (lu2 - L2)
+    return float(np.sqrt(1.0 + acc))
+
+
+# --------------------------------------------------------------------------
+# Keldysh rate (D3/sim/keldysh.py:80-148)
+# --------------------------------------------------------------------------
+class Keldysh:
+    """W_PI(I) in cm^-3 s^-1 following keldysh.py:94-129 (``rate_raw``) and the
+    optional monotone envelope of keldysh.py:132-148."""
+
+    def __init__(self, wavelength, Ui_eV, meff, n0, N_sum=60, monotone=False,
+                 beta_den=4.0, lut_logI=(0.0, 17.0), lut_points=6000):
+        self.U = Ui_eV * q_e
+        self.meff = meff
+        self.n0 = n0
+        self.omega = 2 * np.pi * c / wavelength
+        self.N_sum = N_sum
+        self.monotone = monotone
+        self.beta_den = float(beta_den)
+        self.lut_logI = lut_logI
+        self.lut_points = lut_points
+        self._mono = None
+
+    def rate_raw(self, I_Wcm2):
+        I = np.asarray(I_Wcm2, dtype=np.float64) * 1e4               # :96
+        E = np.sqrt(2.0 * I / (self.n0 * c * epsilon_0))              # :97
+        g = np.maximum(self.omega * np.sqrt(self.meff * self.U)
+                       / (q_e * np.maximum(E, 1e-300)), 1e-12)        # :98
+        G = g * g / (1.0 + g * g)                                     # :100
+        X = 1.0 / (1.0 + g * g)                                       # :101
+        sG = np.sqrt(G)
+        KG, EG = ellipk(G), ellipe(G)                                 # :104
+        KX, EX = ellipk(X), ellipe(X)                                 # :105
+        x = 2.0 * self.U * EX / (np.pi * sG * HBAR * self.omega)      # :107
+        mx = np.floor(x + 1.0)                                        # :108
+        nu = mx - x                                                   # :109
+        al = np.pi * (KG - EG) / EX                                   # :110
+        n = np.arange(self.N_sum).reshape((-1,) + (1,) * np.ndim(x))  # :121
+        arg = np.pi * np.sqrt(np.maximum(n + 2.0 * nu, 0.0)
+                              / (self.beta_den * KX * EX))            # :122
+        S = np.sum(np.exp(-n * al) * dawsn(arg), axis=0)              # :123
+        Q = np.sqrt(np.pi / (2.0 * KX)) * S                           # :125
+        pref = (2.0 * self.omega / (9.0 * np.pi)) * \
+            (self.omega * self.meff / (HBAR * sG)) ** 1.5             # :126-127
+        W = pref * Q * np.exp(-al * mx) * 1e-6                        # :128
+        return np.nan_to_num(W, nan=0.0, posinf=0.0, neginf=0.0)
+
+    def rate(self, I_Wcm2, monotone=None):
+        mono = self.monotone if monotone is None else monotone        # :142
+        if not mono:
+            return self.rate_raw(I_Wcm2)
+        if self._mono is None:                                        # :132-138
+            lI = np.linspace(self.lut_logI[0], self.lut_logI[1], self.lut_points)
+            W = np.maximum.accumulate(np.maximum(self.rate_raw(10.0 ** lI), 0.0))
+            self._mono = (lI, PchipInterpolator(lI, W))
+        lI, sp = self._mono
+        I = np.asarray(I_Wcm2, dtype=np.float64)
+        lq = np.log10(np.clip(I, 10.0 ** lI[0], 10.0 ** lI[-1]))
+        return np.maximum(np.nan_to_num(sp(lq), nan=0.0, posinf=0.0, neginf=0.0), 0.0)
+
+
+def keldysh_validate(wavelength=800e-9, Ui_eV=9.0, meff_rel=0.64, sigma6=9.6e-70,
+                     rho_at=2.1e22):
+    """Quantities asserted by keldysh.py:154-246 (returned, not asserted)."""
+    n0 = n_sellmeier(wavelength)
+    kel = Keldysh(wavelength, Ui_eV, meff_rel * m_e, n0)
+    out = {}
+    gb = 100.0
+    G, X = gb * gb / (1 + gb * gb), 1.0 / (1 + gb * gb)
+    out["beta_mpi_limit"] = float(np.pi ** 2 / (4.0 * ellipk(X) * ellipe(X)))
+    a_inf = np.pi * (ellipk(G) - ellipe(G)) / ellipe(X)
+    out["exp_alpha_ratio"] = float(np.exp(-a_inf) / (np.e ** 2 / (16 * gb * gb)))
+    I_lo = np.array([5e11, 1e12, 2e12])
+    out["mpi_ratio"] = kel.rate(I_lo) / (sigma6 * I_lo ** 6 * rho_at)
+    out["order_low_I"] = np.diff(np.log(kel.rate(I_lo))) / np.diff(np.log(I_lo))
+    E = np.sqrt(2 * 3.5e13 * 1e4 / (n0 * c * epsilon_0))
+    out["gamma_at_3.5e13"] = float(2 * np.pi * c / wavelength
+                                   * np.sqrt(meff_rel * m_e * Ui_eV * q_e) / (q_e * E))
+    out["W_3.5e13_beta4"] = float(kel.rate(np.array([3.5e13]))[0])
+    kel2 = Keldysh(wavelength, Ui_eV, meff_rel * m_e, n0, beta_den=2.0)
+    out["W_3.5e13_beta2"] = float(kel2.rate(np.array([3.5e13]))[0])
+    I = np.logspace(11.5, 14.2, 600)
+    Wr = kel.rate_raw(I)
+    sr = np.diff(np.log(np.maximum(Wr, 1e-300))) / np.diff(np.log(I))
+    out["n_decreasing_raw"] = int((sr < 0).sum())
+    out["slope_min_raw"] = float(sr.min())
+    return out
+
+
+# --------------------------------------------------------------------------
+# Config (D3/sim/config.py:51-255)
+# --------------------------------------------------------------------------
+@dataclass
+class Config:
+    nz: int = 40000
+    Nt: int = 4096
+    N: int = 1025
+    begin: float = -600e-6
+    end: float = 7400e-6
+    R_factor: float = 40.0
+    save_stride: int = 1
+    ckpt_every: int = 500
+    verbose: bool = True
+    out_dir: Optional[str] = None
+    wavelength: float = 1030e-9
+    energy_uJ: float = 15.0
+    peak_power_W: Optional[float] = None
+    w0: float = 10e-6
+    delta_t: float = 263e-15
+    z_focus_air_um: Optional[float] = None
+    n2: float = 2.4e-20
+    Ui_eV: float = 9.0
+    meff_rel: float = 0.64
+    meff_drude_rel: float = 1.0
+    tau_c: float = 1.7e-15
+    tau_r: float = 330e-15
+    f_R: float = 0.18
+    tau_d: float = 32e-15
+    tau_s: float = 12e-15
+    rho_max: float = 2.1e22
+    Us_eV: float = 6.0
+    enable_ste: bool = True
+    tau_ste: Optional[float] = None
+    enable_kerr_instantaneous: bool = True
+    enable_kerr_raman: bool = True
+    enable_self_steepening: bool = True
+    enable_photoionization_loss: bool = True
+    enable_plasma_defocusing: bool = True
+    enable_plasma_absorption: bool = True
+    enable_space_time_focusing: bool = True
+    enable_spectral_filter: bool = True
+    enable_ste_index: bool = True
+    E_tr_eV: float = 4.2
+    enable_avalanche: bool = True
+    enable_recombination: bool = True
+    lambda_probe: float = 490e-9
+    rho_t_stride: int = 15
+    rho_r_stride: int = 1
+    absorb_time_bins_fs: Optional[tuple] = None
+    rho_snapshot_t_fs: Optional[float] = None
+    adaptive: bool = False
+    dphi_max: float = 0.05
+    drho_rel_max: float = 0.10
+    dz_min: float = 10e-9
+    dz_max: float = 200e-9
+    dz_init: Optional[float] = None
+    # EXTENSIONS (defaults reproduce the reference)
+    t_window: Optional[float] = None
+    write_output: bool = True
+
+    def __post_init__(self):                                          # config.py:202
+        self.omega0 = 2 * np.pi * c / self.wavelength
+        self.omega_probe = 2 * np.pi * c / self.lambda_probe
+        self.n0_probe = n_sellmeier(self.lambda_probe)
+        self.nc_probe = (epsilon_0 * m_e * self.omega_probe ** 2 / q_e ** 2) * 1e-6
+        self.frequency = c / self.wavelength
+        self.n0 = n_sellmeier(self.wavelength)
+        if self.z_focus_air_um is not None:                           # :210-215
+            self.begin = -self.n0 * self.z_focus_air_um * 1e-6
+        self.Ui = self.Ui_eV * q_e
+        self.Us = self.Us_eV * q_e
+        self.meff = self.meff_rel * m_e
+        self.meff_drude = self.meff_drude_rel * m_e
+        self.chi3 = 4 / 3 * epsilon_0 * self.n0 ** 2 * c * self.n2    # :221
+        self.Omega_r2 = 1.0 / self.tau_s ** 2 + 1.0 / self.tau_d ** 2
+        self.komega = 2 * np.pi / self.wavelength * self.n0
+        self.dz = (self.end - self.begin) / self.nz                   # :224
+        self.b = self.komega * self.w0 ** 2
+        if self.dz_init is None:
+            self.dz_init = min(self.dz, self.dz_max)
+        if self.peak_power_W is not None:                             # :229-241
+            I0 = 2.0 * self.peak_power_W / (np.pi * self.w0 ** 2)
+        else:
+            tp = self.delta_t / np.sqrt(2 * np.log(2))
+            P = self.energy_uJ * 1e-6 / (tp * np.sqrt(np.pi / 2))
+            I0 = 2.0 * P / (np.pi * self.w0 ** 2)
+        self.I0_Wcm2 = I0 * 1e-4
+        self.E0 = float(np.sqrt(2.0 * self.I0_Wcm2 * 1e4 / (self.n0 * c * epsilon_0)))
+        if self.out_dir is None:
+            self.out_dir = (f"sio2_{datetime.datetime.now():%Y%m%d_%H%M%S}"
+                            f"_E{self.energy_uJ * 1e3:.0f}nJ_{self.delta_t * 1e15:.0f}fs")
+
+
+# --------------------------------------------------------------------------
+# Grids (D3/sim/grids.py:29-185)
+# --------------------------------------------------------------------------
+def keldysh_knots():
+    """Linear-I PCHIP knots, 17 decades x 900 (grids.py:140)."""
+    return np.ravel(10 ** (np.arange(17) + 0.0)[:, None]
+                    * (0.01 * np.arange(900) + 1)[None, :])
+
+
+def build_grids(cfg: Config) -> dict:
+    komega, omega0 = cfg.komega, cfg.omega0
+    tp = cfg.delta_t / np.sqrt(2 * np.log(2))                         # :32
+    tmax = 5 * tp if cfg.t_window is None else 0.5 * cfg.t_window     # :33 (+EXT)
+    dt = float(tmax * 2 / cfg.Nt)                                     # :34
+    tlist = np.linspace(-tmax, tmax, cfg.Nt, endpoint=False)          # :35
+    ff = np.fft.fftfreq(cfg.Nt, d=dt)                                 # :36
+
+    N = cfg.N
+    jz = jn_zeros(0, N)                                               # :39
+    R = cfg.R_factor * cfg.w0
+    ii = np.arange(N - 1)
+    # Y[i, j] = (2/j_N) J0(j_i j_j / j_N) / J1(j_j)^2, grids.py:41-42 (meshgrid 'xy':
+    # idx[0][i, j] = j, idx[1][i, j] = i)
+    jj_col = jz[ii][None, :]
+    jj_row = jz[ii][:, None]
+    Y = (2 / jz[N - 1]) / j1(jj_col) ** 2 * j0(jj_col * jj_row / jz[N - 1])
+    rlist = jz[0:N - 1] * R / jz[N - 1]                               # :43
+    rholist = jz[0:N - 1] / R                                         # :44
+
+    omega_safe = np.clip(omega0 + 2 * np.pi * ff,
+                         2 * np.pi * c / 5.0e-6, 2 * np.pi * c / 0.18e-6)  # :46-47
+    lam_um = (2 * np.pi * c / omega_safe) * 1e6
+    n2m1 = np.zeros_like(lam_um)
+    for B, L2 in zip(SELLMEIER_B, SELLMEIER_L2):
+        n2m1 += B * lam_um ** 2 / (lam_um ** 2 - L2)
+
+    def k_of(w):                                                      # :53-56
+        lu = 2 * np.pi * c / w * 1e6
+        return (w / c) * np.sqrt(1.0 + sum(B * lu ** 2 / (lu ** 2 - L2)
+                                           for B, L2 in zip(SELLMEIER_B, SELLMEIER_L2)))
+    dw = omega0 / 1000.0
+    k1 = (k_of(omega0 + dw) - k_of(omega0 - dw)) / (2 * dw)           # :58
+    delta_k = ((omega_safe / c) * np.sqrt(1.0 + n2m1)
+               - float(komega) - k1 * (2 * np.pi * ff))               # :59-60
+
+    u_norm = (cfg.frequency + ff) / cfg.frequency                     # :65
+    if cfg.enable_spectral_filter:
+        u_lo = (c / 5.0e-6) / cfg.frequency
+        u_hi = (c / 0.18e-6) / cfg.frequency
+        w_edge = 0.05
+        spec_mask = 0.25 * (1.0 + np.tanh((u_norm - u_lo) / w_edge)) \
+            * (1.0 + np.tanh((u_hi - u_norm) / w_edge))               # :70-71
+    else:
+        spec_mask = np.ones_like(ff)
+    T_op = (1.0 + ff / cfg.frequency) * spec_mask if cfg.enable_self_steepening \
+        else np.ones_like(ff)                                         # :79-82
+    U_op = 1.0 + k1 * (2 * np.pi * ff) / komega if cfg.enable_space_time_focusing \
+        else np.ones_like(ff)                                         # :88-91
+    inv_U_op = 1.0 / U_op
+    inv_U_nl = spec_mask / np.maximum(U_op, 0.1) if cfg.enable_space_time_focusing \
+        else np.ones_like(ff)                                         # :108-111
+
+    R_t = np.where(tlist > 0,
+                   cfg.Omega_r2 * cfg.tau_s * np.exp(-tlist / cfg.tau_d) * np.sin(tlist / cfg.tau_s),
+                   np.zeros_like(tlist))
+    R_f = np.fft.fft(np.fft.ifftshift(R_t)) * dt                      # :113-116
+
+    tt, rr = np.meshgrid(tlist, rlist)                                # :118
+    _, rhorho = np.meshgrid(tlist, rholist)                           # :119
+
+    sigmaomega = ((float(komega) * q_e ** 2 * cfg.tau_c) /
+                  (cfg.n0 ** 2 * cfg.meff_drude * epsilon_0 * omega0 *
+                   (1.0 + (omega0 * cfg.tau_c) ** 2))) * 1e4          # :121-123
+    avalanche_coef = (sigmaomega / cfg.Ui) if cfg.enable_avalanche else 0.0
+    avalanche_coef_s = (sigmaomega / cfg.Us if (cfg.enable_ste and cfg.enable_avalanche) else 0.0)
+    inv_taur_eff = (1.0 / cfg.tau_r) if cfg.enable_recombination else 0.0
+    inv_tau_ste = (1.0 / cfg.tau_ste) if (cfg.tau_ste and cfg.enable_ste) else 0.0
+    invE2 = 0.5 * cfg.n0 * c * epsilon_0 * 1e-4                       # :128
+
+    rho_c_pump = (epsilon_0 * m_e * omega0 ** 2 / q_e ** 2) * 1e-6    # :133
+    w_tr = cfg.E_tr_eV * q_e / 1.054571817e-34
+    f_ste = omega0 ** 2 / (w_tr ** 2 - omega0 ** 2)
+    ste_pref = ((omega0 / c) * f_ste / (2.0 * cfg.n0 * rho_c_pump)
+                if (cfg.enable_ste and cfg.enable_ste_index) else 0.0)  # :136-137
+    mask_r = np.exp(-(rr / (0.9 * R)) ** 20)                          # :138
+
+    I_knots = keldysh_knots()                                         # :140
+    W_g = np.maximum(np.nan_to_num(
+        Keldysh(cfg.wavelength, cfg.Ui_eV, cfg.meff, cfg.n0).rate(I_knots)), 0.0)
+    f_g = PchipInterpolator(I_knots, W_g)                             # :144-145
+    W_s = np.maximum(np.nan_to_num(
+        Keldysh(cfg.wavelength, cfg.Us_eV, cfg.meff, cfg.n0).rate(I_knots)), 0.0)
+    f_s = PchipInterpolator(I_knots, W_s)                             # :149-150
+    logI = np.linspace(float(np.log10(I_knots[0])), float(np.log10(I_knots[-1])), 4096)
+    kel = dict(
+        f_spline=f_g,
+        W_LUT=np.nan_to_num(np.abs(f_g(10.0 ** logI)), nan=0.0, posinf=0.0, neginf=0.0),
+        W_LUT_s=np.nan_to_num(np.abs(f_s(10.0 ** logI)), nan=0.0, posinf=0.0, neginf=0.0),
+        NLUT=4096,
+        logImin=float(np.log10(I_knots[0])),
+        inv_dlog=float(4095 / (float(np.log10(I_knots[-1])) - float(np.log10(I_knots[0])))),
+        Imin=float(I_knots[0]), Imax=float(I_knots[-1]),
+        W_knots_g=W_g, W_knots_s=W_s,
+    )
+    return dict(
+        komega=float(komega), tmax=tmax, dt=dt, tlist=tlist, ff=ff,
+        Y=Y, R=R, j1last=jz[N - 1], rlist=rlist, rholist=rholist,
+        delta_k=delta_k, k1=k1, T_op=T_op, inv_U_op=inv_U_op, inv_U_nl=inv_U_nl, R_f=R_f,
+        spec_mask=spec_mask, rr=rr, tt=tt, rhorho=rhorho,
+        sigmaomega=sigmaomega, avalanche_coef=avalanche_coef,
+        avalanche_coef_s=avalanche_coef_s,
+        inv_taur_eff=inv_taur_eff, inv_tau_ste=inv_tau_ste,
+        ste_pref=ste_pref, f_ste=f_ste, invE2=invE2, mask_r=mask_r,
+        b=cfg.b, E0=cfg.E0, keldysh=kel,
+    )
+
+
+def envelope_gaussian_focused(rr, tt, cfg, g):
+    """grids.py:180-183."""
+    tp = cfg.delta_t / np.sqrt(2 * np.log(2))
+    curv = 1 + 1j * 2 * cfg.begin / g["b"]
+    return (g["E0"] / curv) * np.exp(-rr ** 2 / cfg.w0 ** 2 / curv) * np.exp(-tt ** 2 / tp ** 2)
+
+
+ENVELOPES = {"gaussian_focused": envelope_gaussian_focused}
+
+
+def seeded_noise(shape, seed):
+    """xi = (x + i y)/sqrt(2), x, y ~ N(0, 1) from default_rng(seed), drawn as
+    one (Nr, Nt, 2) float64 block (the workload noise model)."""
+    xy = np.random.default_rng(seed).standard_normal(tuple(shape) + (2,))
+    return (xy[..., 0] + 1j * xy[..., 1]) / np.sqrt(2.0)
+
+
+def envelope_lens_noise(lens_f=None, noise=0.0, seed=0):
+    """EXTENSION: gaussian_focused envelope x thin-lens phase exp(-i k0 r^2/(2 f))
+    x (1 + noise * xi). Same convention as grids.py:182 (converging for f > 0)."""
+    def fn(rr, tt, cfg, g):
+        u = envelope_gaussian_focused(rr, tt, cfg, g)
+        if lens_f:
+            u = u * np.exp(-1j * g["komega"] * rr ** 2 / (2.0 * lens_f))
+        if noise:
+            u = u * (1.0 + noise * seeded_noise(u.shape, seed))
+        return u
+    return fn
+
+
+# --------------------------------------------------------------------------
+# Operators (D3/sim/operators.py:22-183)
+# --------------------------------------------------------------------------
+class LinearOperator:
+    def __init__(self, cfg, g):                                       # :35-45
+        self.komega = g["komega"]
+        self.Y = g["Y"]
+        self.Rk = g["R"] ** 2 / g["j1last"]
+        self.iRk = g["j1last"] / g["R"] ** 2
+        self.rhorho = g["rhorho"]
+        self.delta_k = g["delta_k"]
+        self.inv_U_op = g["inv_U_op"]
+        self.spec_mask = g["spec_mask"]
+
+    def half_linear(self, u, dz):                                     # :47-53
+        psik = self.Rk * np.dot(self.Y.astype(np.complex128), u)
+        psik_f = np.fft.fft(psik, axis=1)
+        phase = (self.delta_k - self.rhorho ** 2 / (2 * self.komega) * self.inv_U_op) * dz / 2
+        psik_f = psik_f * np.exp(1j * phase) * self.spec_mask
+        psik = np.fft.ifft(psik_f, axis=1)
+        return self.iRk * np.dot(self.Y.astype(np.complex128), psik)
+
+
+def pchip_eval(spline, x):
+    return spline(x)
+
+
+class NonlinearOperator:
+    def __init__(self, cfg, g):                                       # :59-87
+        self.n0, self.Ui, self.f_R = cfg.n0, cfg.Ui, cfg.f_R
+        self.T_op, self.R_f, self.invE2 = g["T_op"], g["R_f"], g["invE2"]
+        self.inv_U_nl = g["inv_U_nl"]
+        self.kerr_pref = 3 * cfg.chi3 * cfg.omega0 ** 2 / (8 * g["komega"] * c ** 2)
+        self.plasma_pref = (g["sigmaomega"] / 2.0) * 100.0
+        self.plasma_phase = 1.0 + 1j * cfg.omega0 * cfg.tau_c
+        self.en_kerr_inst = bool(cfg.enable_kerr_instantaneous)
+        self.en_kerr_raman = bool(cfg.enable_kerr_raman)
+        self.en_pi_loss = bool(cfg.enable_photoionization_loss)
+        self.en_plasma_defoc = bool(cfg.enable_plasma_defocusing)
+        self.en_plasma_absorb = bool(cfg.enable_plasma_absorption)
+        self.ste_pref = g["ste_pref"]
+        kel = g["keldysh"]
+        self.f_spline = kel["f_spline"]
+        self.Imin, self.Imax = kel["Imin"], kel["Imax"]
+        self._kel = kel
+        self.avalanche = g["avalanche_coef"]
+        self.avalanche_s = g["avalanche_coef_s"]
+        self.inv_taur = g["inv_taur_eff"]
+        self.inv_tau_ste = g["inv_tau_ste"]
+        self.rho_max = cfg.rho_max
+        self.enable_ste = int(cfg.enable_ste)
+
+    def _photo(self, absu2, rho):
+        W = np.nan_to_num(np.abs(self.f_spline(np.clip(absu2 * self.invE2, self.Imin, self.Imax))),
+                          nan=0.0, posinf=0.0, neginf=0.0)            # :92-94
+        depl = np.clip(1.0 - rho / self.rho_max, 0.0, 1.0)            # :96
+        photo = (W * 1e6) * self.Ui / (self.n0 * c * epsilon_0 * (absu2 + 1e-30)) * depl  # :97
+        if not self.en_pi_loss:
+            photo = photo * 0.0                                       # :98-102
+        return photo
+
+    def split(self, u, rho, rho_s=None):                              # :89-146
+        u = np.ascontiguousarray(u.astype(np.complex128, copy=False))
+        absu2 = np.abs(u) ** 2
+        photo = self._photo(absu2, rho)
+        alpha = photo
+        if self.en_plasma_absorb:
+            alpha = alpha + self.plasma_pref * rho
+        alpha = np.maximum(np.nan_to_num(alpha, nan=0.0, posinf=0.0, neginf=0.0), 0.0)
+        kerr_I = np.zeros_like(absu2)
+        if self.en_kerr_inst:
+            kerr_I = kerr_I + (1.0 - self.f_R) * absu2
+        if self.en_kerr_raman:
+            kerr_I = kerr_I + self.f_R * np.fft.ifft(np.fft.fft(absu2, axis=1) * self.R_f, axis=1).real
+        plasma_coeff = ((1.0 if self.en_plasma_absorb else 0.0)
+                        + ((self.plasma_phase - 1.0) if self.en_plasma_defoc else 0.0))
+        extra = -self.plasma_pref * plasma_coeff * rho * u
+        if self.ste_pref and rho_s is not None:
+            extra = extra + 1j * self.ste_pref * rho_s * u
+        NL = (np.fft.fft(1j * self.kerr_pref * kerr_I * u, axis=1) * self.T_op ** 2
+              - np.fft.fft(photo * u, axis=1) * self.T_op
+              + np.fft.fft(extra, axis=1))                            # :141-143
+        rhs = np.fft.ifft(NL * self.inv_U_nl, axis=1) + alpha * u    # :144
+        rhs = np.nan_to_num(np.where(absu2 < 1e-30, 0.0 + 0.0j, rhs),
+                            nan=0.0, posinf=0.0, neginf=0.0)          # :145
+        return rhs, alpha
+
+    def alpha_only(self, u, rho):
+        """The alpha output of split() without the FFT work (split's alpha does
+        not depend on the FFT branch)."""
+        absu2 = np.abs(u) ** 2
+        alpha = self._photo(absu2, rho)
+        if self.en_plasma_absorb:
+            alpha = alpha + self.plasma_pref * rho
+        return np.maximum(np.nan_to_num(alpha, nan=0.0, posinf=0.0, neginf=0.0), 0.0)
+
+    def loss_rates(self, u, rho):                                     # :148-163
+        absu2 = np.abs(u) ** 2
+        photo = self._photo(absu2, rho)
+        plasma = self.plasma_pref * rho if self.en_plasma_absorb else np.zeros_like(photo)
+        return photo, plasma
+
+    def update_plasma(self, u, rho, rho_s, dt):                       # :165-183
+        rho[:, 0] = 0.0
+        rho_s[:, 0] = 0.0
+        rate_equation_rows(u, rho, rho_s, dt, self.invE2, self.avalanche, self.avalanche_s,
+                           self.rho_max, self.inv_taur, self.inv_tau_ste, self.enable_ste,
+                           self._kel["W_LUT"], self._kel["W_LUT_s"], self._kel["NLUT"],
+                           self._kel["logImin"], self._kel["inv_dlog"],
+                           self._kel["Imin"], self._kel["Imax"])
+
+
+# --------------------------------------------------------------------------
+# Rate-equation kernel (D3/sim/kernels.py:13-106), vectorised over rows
+# --------------------------------------------------------------------------
+def lut_rate(I, table, size, log_min, inv_log_step, Imin, Imax):
+    """interpolate_pi_rate, kernels.py:14-26."""
+    I = np.where(~np.isfinite(I) | (I < 0.0), 0.0, I)
+    I = np.minimum(np.maximum(I, Imin), Imax)
+    fidx = (np.log10(I) - log_min) * inv_log_step
+    idx = np.clip(fidx.astype(np.int64), 0, size - 2)                 # (int) truncation
+    r = table[idx] + (fidx - idx) * (table[idx + 1] - table[idx])
+    return np.where(~np.isfinite(r) | (r < 0.0), 0.0, r)
+
+
+def exact_exp_step(x, S, L, dt):
+    """kernels.py:28-34 (note (exp(x)-1)/x, not expm1, as the kernel has it)."""
+    Ldt = L * dt
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        e = np.exp(Ldt)
+        phi1 = np.where(np.abs(Ldt) > 1e-6, (e - 1.0) / np.where(Ldt == 0, 1.0, Ldt),
+                        1.0 + Ldt * (0.5 + Ldt * (1.0 / 6.0 + Ldt / 24.0)))
+        xn = e * x + S * dt * phi1
+    return np.where(~np.isfinite(xn) | (xn < 0.0), 0.0, xn)
+
+
+def rate_equation_rows(E, ne, ns, dt, field_to_I, beta_g, beta_s, na, inv_tau_r,
+                       inv_tau_ste, enable_ste, W_val, W_ste, size, log_min, inv_step,
+                       Imin, Imax):
+    """solve_rate_equation_kernel, kernels.py:36-106: one row per radius,
+    sequential in t; rows advance together. Writes ne[:, 1:], ns[:, 1:]."""
+    Nr, Nt = E.shape
+    ne_v = ne[:, 0].copy()
+    ns_v = ns[:, 0].copy()
+    ne_v = np.where(~np.isfinite(ne_v) | (ne_v < 0.0), 0.0, ne_v)    # :53-55
+    ns_v = np.where(~np.isfinite(ns_v) | (ns_v < 0.0), 0.0, ns_v)
+    I_all = (E.real ** 2 + E.imag ** 2) * field_to_I                  # f.x*f.x + f.y*f.y
+    lut = (size, log_min, inv_step, Imin, Imax)
+    W_all = lut_rate(I_all, W_val, *lut)
+    Ws_all = lut_rate(I_all, W_ste, *lut) if enable_ste else np.zeros_like(I_all)
+    for k in range(Nt - 1):                                           # :62
+        I_c, I_n = I_all[:, k], I_all[:, k + 1]
+        Ia = 0.5 * (I_c + I_n)
+        Wa = 0.5 * (W_all[:, k] + W_all[:, k + 1])
+        Wsa = 0.5 * (Ws_all[:, k] + Ws_all[:, k + 1])
+        depl = np.minimum(np.maximum(1.0 - ne_v / na, 0.0), 1.0)      # :73-75
+        if enable_ste:
+            Se = Wa * depl + (Wsa + beta_s * Ia * ne_v) * (ns_v / na)  # :77
+        else:
+            Se = Wa * depl
+        Le = beta_g * Ia * depl - inv_tau_r                           # :78
+        ne_n = exact_exp_step(ne_v, Se, Le, dt)                       # :88
+        if enable_ste:
+            Ss = inv_tau_r * ne_v                                     # :80
+            Ls = -(Wsa + beta_s * Ia * ne_v) / na - inv_tau_ste       # :86
+            ns_n = exact_exp_step(ns_v, Ss, Ls, dt)
+        else:
+            ns_n = np.zeros_like(ne_n)
+        tot = ne_n + ns_n
+        over = tot > na                                               # :91-95
+        with np.errstate(divide="ignore", invalid="ignore"):
+            scale = np.where(over, na / np.where(over, tot, 1.0), 1.0)
+        ne_n = np.where(over, ne_n * scale, ne_n)
+        ns_n = np.where(over, ns_n * scale, ns_n)
+        ne[:, k + 1] = ne_n
+        ns[:, k + 1] = ns_n
+        ne_v, ns_v = ne_n, ns_n
+
+
+# --------------------------------------------------------------------------
+# Integrator (D3/sim/integrator.py:28-340)
+# --------------------------------------------------------------------------
+class Integrator:
+    def __init__(self, cfg: Config, envelope: Union[str, Callable] = "gaussian_focused",
+                 u0: Optional[np.ndarray] = None):
+        self.cfg = cfg
+        if cfg.write_output:
+            os.makedirs(cfg.out_dir, exist_ok=True)
+        g = build_grids(cfg)
+        self.g = g
+        self.dz, self.dt = cfg.dz, g["dt"]
+        self.lin = LinearOperator(cfg, g)
+        self.nl = NonlinearOperator(cfg, g)
+        if u0 is None:
+            fn = envelope if callable(envelope) else ENVELOPES[envelope]
+            u0 = fn(g["rr"], g["tt"], cfg, g)
+        self.u = np.ascontiguousarray(np.asarray(u0).astype(np.complex128, copy=False))
+        self.Nr, self.Nt = self.u.shape
+        self.rho = np.zeros((self.Nr, self.Nt))
+        self.rho_s = np.zeros((self.Nr, self.Nt))
+        self.mask_r = g["mask_r"]
+        fl0 = np.sum(np.abs(self.u) ** 2, axis=1) * self.dt * g["invE2"] * 1e4   # :48
+        self.U0_uJ = float(np.sum(fl0 * 2.0 * np.pi * g["rlist"]
+                                  * np.diff(g["rlist"], prepend=0.0))) * 1e6
+        if cfg.verbose:
+            print(f"[init] U_beam(0) = {self.U0_uJ:.3f} uJ ", flush=True)
+        n_saves = cfg.nz // cfg.save_stride + 1                       # :54
+        self.n_saves = n_saves
+        self.fluence_rz = np.zeros((n_saves, self.Nr), np.float32)
+        self.rho_rz = np.zeros((n_saves, self.Nr), np.float32)
+        self.rho_s_rz = np.zeros((n_saves, self.Nr), np.float32)
+        self.Imax_z = np.zeros(n_saves)
+        self.z_saved = np.zeros(n_saves)
+        self._dEdz_photo_uJm = np.zeros(n_saves)
+        self._dEdz_plasma_uJm = np.zeros(n_saves)
+        self.E_STE_z = np.zeros(n_saves)
+        self.k_save = 0
+        tlist_fs = g["tlist"] * 1e15
+        self.t_full_fs = tlist_fs
+        self._t_stride = max(1, cfg.rho_t_stride) if cfg.rho_t_stride > 0 else 0  # :73
+        self._r_stride = max(1, cfg.rho_r_stride)
+        if self._t_stride > 0:
+            Nt_sub = (self.Nt - 1) // self._t_stride + 1
+            Nr_sub = (self.Nr - 1) // self._r_stride + 1
+            self.rho_rzt = np.zeros((n_saves, Nr_sub, Nt_sub), np.float32)
+            self.rho_s_rzt = np.zeros((n_saves, Nr_sub, Nt_sub), np.float32)
+            self.I_rzt = np.zeros((n_saves, Nr_sub, Nt_sub), np.float32)
+            self.t_sub_fs = tlist_fs[::self._t_stride]
+            self.r_sub = g["rlist"][::self._r_stride]
+        else:
+            self.rho_rzt = self.rho_s_rzt = self.I_rzt = None
+            self.t_sub_fs = self.r_sub = None
+        cube_b = (3 * self.rho_rzt.nbytes) if self.rho_rzt is not None else 0   # :88-90
+        rest_b = 4 * n_saves * (10 * 2 * self.Nr + 3 * self.Nt)
+        self.size_estimate_gb = (cube_b + rest_b) / 1024 ** 3
+        self.size_cube_frac = cube_b / (cube_b + rest_b)
+        self.absorbed_rz = np.zeros((n_saves, self.Nr), np.float32)
+        if cfg.absorb_time_bins_fs:                                   # :115-123
+            edges = np.asarray(cfg.absorb_time_bins_fs, dtype=np.float64)
+            self._absorb_masks = [(tlist_fs >= a) & (tlist_fs < b) for a, b in zip(edges[:-1], edges[1:])]
+            self.absorb_bin_edges_fs = edges
+            self.absorbed_rz_bins = np.zeros((n_saves, len(self._absorb_masks), self.Nr), np.float32)
+        else:
+            self._absorb_masks = self.absorb_bin_edges_fs = self.absorbed_rz_bins = None
+        self.Ipeak_rz = np.zeros((n_saves, self.Nr), np.float32)
+        if cfg.rho_snapshot_t_fs is not None:                         # :128-132
+            self._it_snap = int(np.argmin(np.abs(tlist_fs - cfg.rho_snapshot_t_fs)))
+            self.rho_rz_at = np.zeros((n_saves, self.Nr), np.float32)
+        else:
+            self._it_snap, self.rho_rz_at = None, None
+        self.rho_onaxis_t = np.zeros((n_saves, self.Nt), np.float32)
+        self.rho_s_onaxis_t = np.zeros((n_saves, self.Nt), np.float32)
+        self.I_onaxis_t = np.zeros((n_saves, self.Nt), np.float32)
+
+    def step(self, dz):                                               # :138-149
+        u, rho, rho_s = self.u, self.rho, self.rho_s
+        u = self.lin.half_linear(u, dz)
+        a = self.nl.alpha_only(u, rho); u = u * np.exp(-0.5 * dz * a)
+        k1, _ = self.nl.split(u, rho, rho_s)
+        k2, _ = self.nl.split(u + 0.5 * dz * k1, rho, rho_s)
+        k3, _ = self.nl.split(u + 0.5 * dz * k2, rho, rho_s)
+        k4, _ = self.nl.split(u + dz * k3, rho, rho_s)
+        u = u + dz / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        a = self.nl.alpha_only(u, rho); u = u * np.exp(-0.5 * dz * a)
+        u = self.lin.half_linear(u, dz)
+        self.u = u * self.mask_r
+
+    def advance(self, i):
+        """One iteration of the propagate() loop body (integrator.py:157-162)."""
+        cfg = self.cfg
+        self.nl.update_plasma(self.u, self.rho, self.rho_s, self.dt)
+        self.step(self.dz)
+        z_now = cfg.begin + i * self.dz
+        if i % cfg.save_stride == 0:
+            self._record(z_now)
+
+    def propagate(self, n_steps=None):                                # :151-177
+        n = self.cfg.nz + 1 if n_steps is None else n_steps
+        for i in range(n):
+            self.advance(i)
+        return self._results()
+
+    def _record(self, z_now):                                         # :179-224
+        k, g = self.k_save, self.g
+        absu2 = np.abs(self.u) ** 2
+        I_full = absu2 * g["invE2"]
+        self.fluence_rz[k] = (np.sum(absu2, axis=1) * self.dt * g["invE2"]).astype(np.float32)
+        self.rho_rz[k] = np.max(self.rho, axis=1).astype(np.float32)
+        self.rho_s_rz[k] = np.max(self.rho_s, axis=1).astype(np.float32)
+        self.Imax_z[k] = float(I_full.max())
+        self.z_saved[k] = float(z_now)
+        photo_al, plasma_al = self.nl.loss_rates(self.u, self.rho)
+        dep = 2.0 * (photo_al + plasma_al) * I_full * self.dt * 100.0
+        self.absorbed_rz[k] = np.sum(dep, axis=1).astype(np.float32)
+        if self._absorb_masks is not None:
+            for b, m in enumerate(self._absorb_masks):
+                self.absorbed_rz_bins[k, b] = np.sum(dep * m, axis=1).astype(np.float32)
+        self.Ipeak_rz[k] = np.max(I_full, axis=1).astype(np.float32)
+        if self._it_snap is not None:
+            self.rho_rz_at[k] = self.rho[:, self._it_snap].astype(np.float32)
+        r, dr = g["rlist"], np.diff(g["rlist"], prepend=0.0)
+        fp = np.sum(2.0 * photo_al * I_full, axis=1) * self.dt
+        fq = np.sum(2.0 * plasma_al * I_full, axis=1) * self.dt
+        self._dEdz_photo_uJm[k] = float(np.sum(fp * 2.0 * np.pi * r * dr)) * 1e4 * 1e6
+        self._dEdz_plasma_uJm[k] = float(np.sum(fq * 2.0 * np.pi * r * dr)) * 1e4 * 1e6
+        if self._t_stride > 0 and self.rho_rzt is not None:
+            rs, ts = self._r_stride, self._t_stride
+            self.rho_rzt[k] = self.rho[::rs, ::ts].astype(np.float32)
+            self.rho_s_rzt[k] = self.rho_s[::rs, ::ts].astype(np.float32)
+            self.I_rzt[k] = I_full[::rs, ::ts].astype(np.float32)
+        self.rho_onaxis_t[k] = self.rho[0].astype(np.float32)
+        self.rho_s_onaxis_t[k] = self.rho_s[0].astype(np.float32)
+        self.I_onaxis_t[k] = I_full[0].astype(np.float32)
+        self.k_save += 1
+
+    def _cumulative_energy_fraction(self, dEdz):                      # :226-236
+        z = self.z_saved[:self.k_save]
+        d = dEdz[:self.k_save]
+        if len(z) < 2:
+            return np.zeros_like(d)
+        seg = 0.5 * (d[:-1] + d[1:]) * np.diff(z)
+        return np.concatenate([[0.0], np.cumsum(seg)]) / self.U0_uJ
+
+    def _results(self):                                               # :238-298
+        k = self.k_save
+        mirror = lambda a: np.hstack([a[:, ::-1], a])                 # noqa: E731
+        r = self.g["rlist"]
+        E_MPI = self._cumulative_energy_fraction(self._dEdz_photo_uJm)
+        E_pl = self._cumulative_energy_fraction(self._dEdz_plasma_uJm)
+        out = dict(
+            r=np.concatenate([-r[::-1], r]), rlist=r, z=self.z_saved[:k],
+            fluence_rz=mirror(self.fluence_rz[:k]), rho_rz=mirror(self.rho_rz[:k]),
+            rho_s_rz=mirror(self.rho_s_rz[:k]), Imax_z=self.Imax_z[:k],
+            E_plasma_z=E_pl, E_MPI_z=E_MPI, E_STE_z=self.E_STE_z[:k], E_total_z=E_MPI + E_pl,
+            rho_rzt=self.rho_rzt[:k] if self.rho_rzt is not None else None,
+            rho_s_rzt=self.rho_s_rzt[:k] if self.rho_s_rzt is not None else None,
+            I_rzt=self.I_rzt[:k] if self.I_rzt is not None else None,
+            t_sub_fs=self.t_sub_fs, r_sub=self.r_sub,
+            absorbed_rz=mirror(self.absorbed_rz[:k]), Ipeak_rz=mirror(self.Ipeak_rz[:k]),
+            rho_rz_at=mirror(self.rho_rz_at[:k]) if self.rho_rz_at is not None else None,
+            absorbed_rz_bins=(np.stack([mirror(self.absorbed_rz_bins[:k, b])
+                                        for b in range(self.absorbed_rz_bins.shape[1])], axis=1)
+                              if self.absorbed_rz_bins is not None else None),
+            absorb_bin_edges_fs=self.absorb_bin_edges_fs,
+            rho_onaxis_t=self.rho_onaxis_t[:k], rho_s_onaxis_t=self.rho_s_onaxis_t[:k],
+            I_onaxis_t=self.I_onaxis_t[:k], t_full_fs=self.t_full_fs,
+        )
+        if self.cfg.write_output:
+            final = os.path.join(self.cfg.out_dir, "result.npz")
+            np.savez_compressed(final, **out)
+            with open(os.path.join(self.cfg.out_dir, "params.json"), "w") as f:
+                json.dump(dict(n0=self.cfg.n0), f)
+        return out
+
+
+def run(*, Nz, Nt, Nr, wavelength, energy_uJ=15.0, peak_power_W=None,
+        envelope="gaussian_focused", w0=10e-6, delta_t=263e-15, begin=-600e-6,
+        end=1800e-6, n2=3.54e-20, Ui_eV=9.0, R_factor=100.0, save_stride=1,
+        ckpt_every=500, verbose=True, out_dir=None, **material):
+    """filament_sim.py:59-82 (defaults differ from Config's on purpose, as there)."""
+    cfg = Config(nz=Nz, Nt=Nt, N=Nr, wavelength=wavelength, energy_uJ=energy_uJ,
+                 peak_power_W=peak_power_W, w0=w0, delta_t=delta_t, begin=begin, end=end,
+                 n2=n2, Ui_eV=Ui_eV, R_factor=R_factor, save_stride=save_stride,
+                 ckpt_every=ckpt_every, verbose=verbose, out_dir=out_dir, **material)
+    return Integrator(cfg, envelope=envelope).propagate()
+
+
+# What pins each part of this restatement (mirrored in tests/test_oracle_pins.py).
+PINS = {
+    "n_sellmeier": "n0(800nm)=1.4533, n0(1030nm)=1.45004 (reference records)",
+    "Keldysh.rate_raw": "validate() asserts + recorded W/W_MPI 1.062/1.018/0.911, order 5.89, "
+                        "W(3.5e13)=6.44e32 (beta_den 4) / 4.30e32 (beta_den 2), 23 decreasing points",
+    "Config.__post_init__": "U_beam(0)=12.561 uJ KAT at the 1030 nm production config",
+    "build_grids.Y": "closed forms: Y@Y=I, Gaussian Hankel pair, non-symmetry",
+    "build_grids.delta_k/k1": "k''(800nm)=361.6 fs^2/cm from delta_k finite differences",
+    "build_grids.R_f": "closed-form DFT of the sampled damped sine",
+    "build_grids.ste_pref": "Delta n_STE(1e20)=3.1131e-3 (reference record)",
+    "LinearOperator": "Gaussian beam q-parameter diffraction, chirped-pulse dispersion, energy",
+    "NonlinearOperator.split": "toggles-off => rhs=alpha=0; Kerr-only => i kappa |u|^2 u; "
+                               "photo/alpha pointwise closed forms; T-hat off => T=1",
+    "rate_equation_rows": "constant-I exact solution of dx/dt=S+Lx; C restatement; "
+                          "non-negativity and rho<=rho_max",
+    "Integrator": "linear-only run = linear Gaussian propagation; nz+1 steps, z labels; "
+                  "record invariants (bins sum, rho_at<=rho_rz, max Ipeak = max Imax); "
+                  "result.npz size estimate 0.99 GB / 86% cube at the 1030 nm config",
+}

@@ -1,0 +1,344 @@
+"""Pins for the CPU oracle (oracle/reference_numpy.py).
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails it: recorded outputs
+of the reference (D3 = .github/deliverable (3)), closed forms, invariants and
+an independent scalar restatement on tiny inputs. No GPU.
+"""
+import numpy as np
+import pytest
+from scipy.constants import c, epsilon_0, m_e
+from scipy.constants import elementary_charge as q_e
+
+from oracle import reference_numpy as O
+
+
+# ---------------------------------------------------------------- Sellmeier
+def test_sellmeier_recorded_values():
+    # D3/ANALYSE_CRITIQUE.md:58 (1.4533 at 800 nm);
+    # D3/notebooks/filament_visualization_1030nm.ipynb:126 (1.45004 at 1030 nm)
+    assert round(O.n_sellmeier(800e-9), 4) == 1.4533
+    assert round(O.n_sellmeier(1030e-9), 5) == 1.45004
+
+
+def test_gvd_from_delta_k():
+    # k''(800 nm) = 361.6 fs^2/cm (D3/ANALYSE_CRITIQUE.md:59), read off the
+    # curvature of build_grids' delta_k around Omega = 0 (delta_k = k - k0 - k1 Omega).
+    cfg = O.Config(nz=1, Nt=4096, N=9, wavelength=800e-9, delta_t=85e-15, w0=100e-6,
+                   R_factor=10, t_window=20e-12, write_output=False, verbose=False)
+    g = O.build_grids(cfg)
+    Om = 2 * np.pi * g["ff"]
+    sel = np.abs(Om) < 2 * np.pi * 2e12
+    p = np.polyfit(Om[sel], g["delta_k"][sel], 3)
+    k2 = 2 * p[1]                                # s^2/m
+    assert abs(k2 * 1e30 / 100 - 361.6) < 0.5    # fs^2/cm
+    assert abs(p[2]) < 1e-6 * abs(g["k1"])       # linear term removed by k1 (central diff)
+
+
+# ---------------------------------------------------------------- Keldysh
+def test_keldysh_recorded_validation_beta_den_2():
+    # D3/ANALYSE_CRITIQUE.md:165-175 was recorded with beta_den = 2 (:158)
+    n0 = O.n_sellmeier(800e-9)
+    k = O.Keldysh(800e-9, 9.0, 0.64 * m_e, n0, beta_den=2.0)
+    I = np.array([5e11, 1e12, 2e12])
+    ratio = k.rate(I) / (9.6e-70 * I ** 6 * 2.1e22)
+    assert np.allclose(np.round(ratio, 3), [1.062, 1.018, 0.911])
+    order = np.diff(np.log(k.rate(I))) / np.diff(np.log(I))
+    assert round(order.mean(), 2) == 5.89
+    II = np.logspace(11.5, 14.2, 600)
+    s = np.diff(np.log(np.maximum(k.rate_raw(II), 1e-300))) / np.diff(np.log(II))
+    assert int((s < 0).sum()) == 23
+    assert round(float(s.min()), 2) == -136.49
+
+
+def test_keldysh_validate_asserts_and_beta_values():
+    v = O.keldysh_validate()
+    # keldysh.py:241-245
+    assert 0.8 < v["mpi_ratio"][1] < 1.25
+    assert 0.95 < v["gamma_at_3.5e13"] < 1.05
+    assert abs(v["beta_mpi_limit"] - 1.0) < 1e-3
+    assert abs(v["exp_alpha_ratio"] - 1.0) < 1e-3
+    # D3/ANALYSE_CRITIQUE.md:172 / SURVEY: W(3.5e13) = 6.44e32 (4) and 4.30e32 (2)
+    assert f"{v['W_3.5e13_beta4']:.2e}" == "6.44e+32"
+    assert f"{v['W_3.5e13_beta2']:.2e}" == "4.30e+32"
+    # W_MPI(3.5e13) = 3.71e34 (D3/ANALYSE_CRITIQUE.md:65)
+    assert f"{9.6e-70 * 3.5e13 ** 6 * 2.1e22:.2e}" == "3.71e+34"
+
+
+def test_keldysh_multiphoton_slope_and_positivity():
+    n0 = O.n_sellmeier(800e-9)
+    k = O.Keldysh(800e-9, 9.0, 0.64 * m_e, n0)
+    I = O.keldysh_knots()
+    W = k.rate(I)
+    assert np.all(np.isfinite(W)) and np.all(W >= 0)
+    # K = ceil(Ui / hbar w0) = 6 at 800 nm: low-intensity log-slope ~ 6
+    lo = (I > 1e10) & (I < 1e11)
+    sl = np.polyfit(np.log(I[lo]), np.log(W[lo]), 1)[0]
+    assert 5.5 < sl < 6.5
+
+
+# ---------------------------------------------------------------- Config KAT
+def _cfg_1030():
+    T = 1.0 - ((1.45 - 1.0) / (1.45 + 1.0)) ** 2
+    n0 = O.n_sellmeier(1030e-9)
+    nz = int((800e-6 + n0 * 272e-6) / 24e-9)
+    return O.Config(nz=nz, Nt=4096, N=3001, R_factor=90.0, end=800e-6, z_focus_air_um=272.0,
+                    save_stride=100, wavelength=1030e-9, energy_uJ=13.0 * T, w0=3e-6,
+                    delta_t=263e-15, n2=2.74e-20, Ui_eV=9.0, meff_rel=0.64, tau_c=1.7e-15,
+                    tau_r=330e-15, rho_max=2.1e22, Us_eV=6.0, f_R=0.18, tau_d=32e-15,
+                    tau_s=12e-15, enable_ste=True, lambda_probe=490e-9, rho_t_stride=20,
+                    rho_r_stride=4, write_output=False, verbose=False)
+
+
+def test_energy_normalisation_kat_1030nm():
+    # filament_visualization_1030nm.ipynb:646 "[init] U_beam(0) = 12.561 uJ",
+    # :647 "result.npz ~ 0.99 GB non compresse (86 % = cube (z,r,t))", :127-128 Nz, begin
+    cfg = _cfg_1030()
+    assert cfg.nz == 49767
+    assert f"{cfg.begin * 1e6:+.2f}" == "-394.41"
+    it = O.Integrator(cfg)
+    assert f"{it.U0_uJ:.3f}" == "12.561"
+    assert f"{it.size_estimate_gb:.2f}" == "0.99"
+    assert round(100 * it.size_cube_frac) == 86
+
+
+# ---------------------------------------------------------------- Hankel matrix
+@pytest.mark.parametrize("N,tol", [(129, 2.4e-10), (513, 3.9e-12)])
+def test_qdht_involution(N, tol):
+    cfg = O.Config(nz=1, Nt=8, N=N, wavelength=800e-9, w0=100e-6, R_factor=10,
+                   write_output=False, verbose=False)
+    Y = O.build_grids(cfg)["Y"]
+    err = np.abs(Y @ Y - np.eye(N - 1)).max()
+    assert err < 1.3 * tol
+    assert np.abs(Y - Y.T).max() > 1e-3       # column J1^2 normalisation: not symmetric
+
+
+def test_qdht_gaussian_pair():
+    # (R^2/j_N) Y exp(-r^2/w^2) = (w^2/2) exp(-rho^2 w^2/4)
+    cfg = O.Config(nz=1, Nt=8, N=257, wavelength=800e-9, w0=100e-6, R_factor=10,
+                   write_output=False, verbose=False)
+    g = O.build_grids(cfg)
+    w = 100e-6
+    f = np.exp(-g["rlist"] ** 2 / w ** 2)
+    F = g["R"] ** 2 / g["j1last"] * g["Y"] @ f
+    ref = w ** 2 / 2 * np.exp(-g["rholist"] ** 2 * w ** 2 / 4)
+    assert np.abs(F - ref).max() / ref.max() < 1e-14
+
+
+# ---------------------------------------------------------------- Raman kernel
+def test_raman_transfer_closed_form():
+    cfg = O.Config(nz=1, Nt=256, N=9, wavelength=800e-9, delta_t=85e-15, w0=100e-6,
+                   R_factor=10, t_window=800e-15, write_output=False, verbose=False)
+    g = O.build_grids(cfg)
+    N, dt = cfg.Nt, g["dt"]
+    A = cfg.Omega_r2 * cfg.tau_s
+    M = N // 2 - 1                       # lags 1..N/2-1 have t > 0 after ifftshift
+    k = np.arange(N)
+    out = np.zeros(N, complex)
+    for sgn in (+1, -1):
+        z = np.exp(-dt / cfg.tau_d + sgn * 1j * dt / cfg.tau_s - 2j * np.pi * k / N)
+        out += sgn * z * (1 - z ** M) / (1 - z) / (2j)
+    out *= A * dt
+    assert np.abs(g["R_f"] - out).max() < 1e-12 * np.abs(out).max()
+
+
+def test_ste_index_prefactor():
+    # Delta n_STE(rho_s = 1e20 cm^-3, 800 nm) = 3.1131e-3 (D3/ANALYSE_CRITIQUE.md:258)
+    cfg = O.Config(nz=1, Nt=8, N=9, wavelength=800e-9, w0=100e-6, write_output=False,
+                   verbose=False)
+    g = O.build_grids(cfg)
+    dn = g["ste_pref"] / (cfg.omega0 / c) * 1e20
+    assert f"{dn:.4e}" == "3.1131e-03"
+
+
+# ---------------------------------------------------------------- linear step
+def _small_cfg(**kw):
+    base = dict(nz=4, Nt=128, N=129, wavelength=800e-9, delta_t=85e-15, w0=100e-6,
+                R_factor=10, begin=0.0, end=4e-3, t_window=800e-15, energy_uJ=2.0,
+                n2=3.2e-20, tau_c=1.27e-15, tau_r=150e-15, enable_ste=False,
+                write_output=False, verbose=False)
+    base.update(kw)
+    return O.Config(**base)
+
+
+@pytest.mark.parametrize("j", [0, 5, -7])
+def test_half_linear_single_frequency_gaussian_beam(j):
+    cfg = _small_cfg(enable_spectral_filter=True)
+    g = O.build_grids(cfg)
+    lin = O.LinearOperator(cfg, g)
+    w0, r, t = cfg.w0, g["rlist"], g["tlist"]
+    fj = g["ff"][j]
+    u = np.exp(-r ** 2 / w0 ** 2)[:, None] * np.exp(2j * np.pi * fj * t)[None, :]
+    dz = 2e-3
+    out = lin.half_linear(u, dz)
+    # effective wavenumber k0 / inv_U(f_j), advance dz/2, plus exp(i dk dz/2) * mask
+    keff = g["komega"] / g["inv_U_op"][j]
+    q = 1 + 2j * (dz / 2) / (keff * w0 ** 2)
+    ref_r = np.exp(-r ** 2 / (w0 ** 2 * q)) / q
+    ref = (ref_r[:, None] * np.exp(2j * np.pi * fj * t)[None, :]
+           * np.exp(1j * g["delta_k"][j] * dz / 2) * g["spec_mask"][j])
+    assert np.abs(out - ref).max() < 1e-12
+
+
+# ---------------------------------------------------------------- split()
+def _field(cfg, g, seed=1, scale=1.0):
+    u = O.envelope_lens_noise(noise=0.05, seed=seed)(g["rr"], g["tt"], cfg, g)
+    return u * scale
+
+
+def test_split_all_toggles_off_is_zero():
+    cfg = _small_cfg(enable_kerr_instantaneous=False, enable_kerr_raman=False,
+                     enable_self_steepening=False, enable_photoionization_loss=False,
+                     enable_plasma_defocusing=False, enable_plasma_absorption=False)
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g)
+    rho = np.full(u.shape, 1e19)
+    rhs, a = nl.split(u, rho, rho)
+    assert np.all(rhs == 0) and np.all(a == 0)
+
+
+def test_split_kerr_only_is_spm():
+    cfg = _small_cfg(enable_kerr_raman=False, enable_self_steepening=False,
+                     enable_photoionization_loss=False, enable_plasma_defocusing=False,
+                     enable_plasma_absorption=False, enable_space_time_focusing=False,
+                     enable_spectral_filter=False)
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g)
+    rhs, _ = nl.split(u, np.zeros(u.shape), None)
+    kappa = 3 * cfg.chi3 * cfg.omega0 ** 2 / (8 * cfg.komega * c ** 2)
+    # kappa |u|^2 = k0 n2 I with I = (n0 c eps0 / 2)|u|^2
+    assert abs(kappa - 2 * np.pi / cfg.wavelength * cfg.n2 * 0.5 * cfg.n0 * c * epsilon_0) < 1e-12 * kappa
+    ref = 1j * (1 - cfg.f_R) * kappa * np.abs(u) ** 2 * u
+    assert np.abs(rhs - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_alpha_pointwise_closed_form():
+    cfg = _small_cfg()
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g, scale=30.0)
+    rho = np.random.default_rng(3).uniform(0, 1e21, u.shape)
+    _, a = nl.split(u, rho, None)
+    I = np.abs(u) ** 2 * g["invE2"]
+    W = g["keldysh"]["f_spline"](np.clip(I, 1.0, 9.99e16))
+    photo = W * 1e6 * cfg.Ui / (cfg.n0 * c * epsilon_0 * (np.abs(u) ** 2 + 1e-30)) * (1 - rho / cfg.rho_max)
+    ref = photo + g["sigmaomega"] / 2 * 100 * rho
+    assert np.abs(a - ref).max() <= 1e-12 * ref.max()
+
+
+# ---------------------------------------------------------------- rate equation
+def _scalar_rate(I, W, Ws, dt, bg, bs, na, itr, its, ste):
+    """Independent scalar restatement of kernels.py:48-105 on a given (I, W, Ws) row."""
+    import math
+    ne = ns = 0.0
+    NE, NS = [0.0], [0.0]
+
+    def step(x, S, L):
+        Ldt = L * dt
+        p = (math.exp(Ldt) - 1.0) / Ldt if abs(Ldt) > 1e-6 else 1.0 + Ldt * (0.5 + Ldt * (1 / 6 + Ldt / 24))
+        xn = math.exp(Ldt) * x + S * dt * p
+        return xn if (math.isfinite(xn) and xn >= 0) else 0.0
+    for k in range(len(I) - 1):
+        Ia, Wa, Wsa = 0.5 * (I[k] + I[k + 1]), 0.5 * (W[k] + W[k + 1]), 0.5 * (Ws[k] + Ws[k + 1])
+        depl = min(max(1.0 - ne / na, 0.0), 1.0)
+        Se = Wa * depl + ((Wsa + bs * Ia * ne) * (ns / na) if ste else 0.0)
+        Le = bg * Ia * depl - itr
+        Ss = itr * ne if ste else 0.0
+        Ls = (-(Wsa + bs * Ia * ne) / na - its) if ste else 0.0
+        a, b = step(ne, Se, Le), (step(ns, Ss, Ls) if ste else 0.0)
+        if a + b > na:
+            s = na / (a + b)
+            a, b = a * s, b * s
+        ne, ns = a, b
+        NE.append(ne)
+        NS.append(ns)
+    return np.array(NE), np.array(NS)
+
+
+@pytest.mark.parametrize("ste", [False, True])
+def test_rate_rows_vs_scalar_restatement(ste):
+    cfg = _small_cfg(Nt=96, N=5, enable_ste=ste, tau_ste=1e-12 if ste else None)
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    rng = np.random.default_rng(7)
+    tt = g["tlist"]
+    amp = np.array([0.5, 1.0, 2.0, 4.0])[:, None] * 6e13
+    I = amp * np.exp(-2 * tt ** 2 / (60e-15) ** 2)[None, :] * (1 + 0.05 * rng.standard_normal((4, cfg.Nt)))
+    u = np.sqrt(np.abs(I) / g["invE2"]).astype(complex)
+    rho, rho_s = np.zeros(u.shape), np.zeros(u.shape)
+    nl.update_plasma(u, rho, rho_s, g["dt"])
+    kel = g["keldysh"]
+    lut = (kel["NLUT"], kel["logImin"], kel["inv_dlog"], kel["Imin"], kel["Imax"])
+    for r in range(4):
+        Ir = (u[r].real ** 2 + u[r].imag ** 2) * g["invE2"]
+        W = O.lut_rate(Ir, kel["W_LUT"], *lut)
+        Ws = O.lut_rate(Ir, kel["W_LUT_s"], *lut) if ste else np.zeros_like(Ir)
+        NE, NS = _scalar_rate(Ir, W, Ws, g["dt"], nl.avalanche, nl.avalanche_s, nl.rho_max,
+                              nl.inv_taur, nl.inv_tau_ste, ste)
+        assert np.allclose(rho[r], NE, rtol=1e-13, atol=0)
+        assert np.allclose(rho_s[r], NS, rtol=1e-13, atol=0)
+    assert rho.max() > 1e15                       # the case exercises the rates
+    assert np.all(rho >= 0) and np.all(rho + rho_s <= cfg.rho_max * (1 + 1e-15))
+
+
+def test_rate_constant_intensity_exact_solution():
+    # Frozen coefficients: no depletion (huge rho_max) and constant I make the
+    # exact-exponential step exact: n(t) = (W/L)(exp(L t) - 1), L = beta I - 1/tau_r.
+    cfg = _small_cfg(Nt=64, N=3, rho_max=1e300)
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    I0 = 3e13
+    u = np.full((2, cfg.Nt), np.sqrt(I0 / g["invE2"]), complex)
+    rho, rho_s = np.zeros(u.shape), np.zeros(u.shape)
+    nl.update_plasma(u, rho, rho_s, g["dt"])
+    kel = g["keldysh"]
+    W = O.lut_rate(np.array([I0]), kel["W_LUT"], kel["NLUT"], kel["logImin"], kel["inv_dlog"],
+                   kel["Imin"], kel["Imax"])[0]
+    Iq = (u[0, 0].real ** 2 + u[0, 0].imag ** 2) * g["invE2"]
+    L = nl.avalanche * Iq - nl.inv_taur
+    t = np.arange(cfg.Nt) * g["dt"]
+    ref = W / L * np.expm1(L * t)
+    assert np.allclose(rho[0], ref, rtol=1e-11, atol=0)
+
+
+# ---------------------------------------------------------------- whole loop
+def test_linear_only_run_follows_gaussian_beam_and_z_labels():
+    # All six field toggles off, STF and spectral filter off: the loop is linear
+    # and fluence obeys the Gaussian-beam law; the state recorded with label
+    # z_i = begin + i dz has propagated (i+1) dz (integrator.py:153-159).
+    cfg = _small_cfg(nz=3, begin=-4e-3, end=-1e-3, N=257, Nt=64, R_factor=15,
+                     enable_kerr_instantaneous=False, enable_kerr_raman=False,
+                     enable_self_steepening=False, enable_photoionization_loss=False,
+                     enable_plasma_defocusing=False, enable_plasma_absorption=False,
+                     enable_space_time_focusing=False, enable_spectral_filter=False,
+                     rho_t_stride=0)
+    it = O.Integrator(cfg)
+    res = it.propagate()
+    assert len(res["z"]) == cfg.nz + 1
+    assert np.allclose(res["z"], cfg.begin + np.arange(cfg.nz + 1) * cfg.dz)
+    g, w0 = it.g, cfg.w0
+    F0 = np.sum(np.abs(np.exp(-g["tlist"] ** 2 / (cfg.delta_t / np.sqrt(2 * np.log(2))) ** 2)) ** 2) \
+        * g["dt"] * g["invE2"] * cfg.E0 ** 2
+    for i, z in enumerate(res["z"]):
+        zp = cfg.begin + (i + 1) * cfg.dz
+        q = 1 + 2j * zp / cfg.b
+        ref = F0 / abs(q) ** 2 * np.exp(-2 * g["rlist"] ** 2 * (1 / (w0 ** 2 * q)).real)
+        Nr = g["rlist"].size
+        got = res["fluence_rz"][i, Nr:]
+        assert np.abs(got - ref).max() < 2e-6 * ref.max()
+
+
+def test_record_invariants():
+    cfg = _small_cfg(nz=2, end=20e-6, N=65, Nt=128, energy_uJ=40.0, w0=40e-6,
+                     absorb_time_bins_fs=(-400, -50, 0, 50, 400),
+                     rho_snapshot_t_fs=20.0, rho_t_stride=3, rho_r_stride=2)
+    res = O.Integrator(cfg).propagate()
+    tot = res["absorbed_rz"]
+    bins = res["absorbed_rz_bins"].sum(axis=1)
+    assert np.abs(bins - tot).max() <= 1e-6 * np.abs(tot).max()
+    assert np.all(res["rho_rz_at"] <= res["rho_rz"])
+    assert np.isclose(res["Ipeak_rz"].max(), res["Imax_z"].max(), rtol=1e-6)
+    assert res["rho_rz"].max() > 0
+    assert res["rho_rzt"].shape == (3, 32, 43)
