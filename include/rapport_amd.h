@@ -1,0 +1,165 @@
+/*
+ * rapport_amd -- C ABI of the MI355X (gfx950) split-step filamentation propagator.
+ *
+ * Plain C: device pointers, sizes and POD parameter blocks; no torch types.
+ * Every device pointer is caller-owned memory on the current HIP device;
+ * `stream` is a hipStream_t (NULL = default stream). Calls are asynchronous on
+ * that stream. Return value: 0 on success, otherwise an FP_E_* code or a
+ * hipError_t value; fp_last_error() gives a message.
+ *
+ * Layouts: fields are row-major (r, t) with interleaved complex (re, im);
+ * densities are row-major (r, t) float64; "spectral" rows hold the numpy-
+ * convention FFT along t of a field row (frequency order of numpy.fft.fftfreq).
+ *
+ * Reference citations are relative to the reference repository;
+ * D3 = ".github/deliverable (3)".
+ */
+#ifndef RAPPORT_AMD_H
+#define RAPPORT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FP_OK 0
+#define FP_E_ARG 9001      /* invalid argument (shape, null pointer, flag) */
+#define FP_E_SHAPE 9002    /* Nt / Nr not supported by the compiled kernels */
+#define FP_E_LAUNCH 9003   /* kernel launch failed */
+
+/* Scalars of the carrier rate kernel, one-to-one with the arguments of
+ * solve_rate_equation_kernel (D3/sim/kernels.py:36-46). */
+typedef struct fp_plasma_params {
+  int Nr, Nt;
+  double dt;
+  double field_to_I;   /* invE2 */
+  double beta_g, beta_s;
+  double na;           /* rho_max */
+  double inv_tau_r, inv_tau_ste;
+  int enable_ste;
+  int table_size;
+  double log_min_I, inv_log_step, Imin, Imax;
+} fp_plasma_params;
+
+/* Scalars of NonlinearOperator.split (D3/sim/operators.py:59-146) and of the
+ * nonlinear part of Integrator.step (D3/sim/integrator.py:141-147). */
+typedef struct fp_nl_params {
+  int Nr, Nt;
+  double dz;
+  double invE2, Imin, Imax;
+  double photo_pref;      /* 1e6 * Ui / (n0 c eps0): photo = W * photo_pref / (|u|^2 + 1e-30) * depl */
+  double rho_max;
+  double plasma_pref;     /* sigma/2 * 100 */
+  double pc_re, pc_im;    /* plasma_coeff (absorption + i omega0 tau_c defocusing, per toggles) */
+  double ste_pref;        /* 0 disables the STE index term */
+  double kerr_pref, f_R;
+  int en_kerr_inst, en_kerr_raman, en_pi_loss, en_plasma_absorb;
+  int n_knots;            /* PCHIP knots: 17 decades x 900, 10^k (1 + 0.01 j) */
+} fp_nl_params;
+
+/* Device tables shared by the nonlinear kernels (float64). */
+typedef struct fp_nl_tables {
+  const double* tw;       /* Nt complex: exp(-2 pi i m / Nt) */
+  const double* mA;       /* Nt: T^2 * invU_nl  (Kerr bracket multiplier)     */
+  const double* mB;       /* Nt: -T * invU_nl   (photo-ionisation multiplier) */
+  const double* mC;       /* Nt: invU_nl        (plasma / STE multiplier)     */
+  const double* Rf;       /* Nt complex: Raman transfer function R_f          */
+  const double* knots;    /* n_knots PCHIP abscissae (W/cm^2)                 */
+  const double* pcoef;    /* 4 x (n_knots-1) PCHIP power-basis coefficients   */
+} fp_nl_tables;
+
+/* Same tables in float32 for the mixed-precision path. PCHIP coefficients are
+ * in the local coordinate s = (I - x_i)/h_i and pre-scaled by photo_pref. */
+typedef struct fp_nl_tables_f32 {
+  const float* tw;
+  const float* mA;
+  const float* mB;
+  const float* mC;
+  const float* Rf;
+  const double* knots;
+  const float* pcoef_s;   /* 4 x (n_knots-1) */
+} fp_nl_tables_f32;
+
+int fp_version(void);
+const char* fp_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Carrier rate equations along t for all rows.
+ * Replaces NonlinearOperator.update_plasma + rate_eq_kernel launch
+ * (D3/sim/operators.py:165-183; kernel D3/sim/kernels.py:36-106).
+ * u: Nr x Nt complex128 (time domain). rho, rho_s: Nr x Nt float64, fully
+ * written (column 0 = 0 as update_plasma sets it). rho_s may be NULL when
+ * enable_ste == 0. status (optional, device int): incremented per row whose
+ * Newton scan hit its iteration cap. Supported Nt: 512, 1024, 2048, 4096.
+ */
+int fp_plasma_scan(const double* u, double* rho, double* rho_s, const fp_plasma_params* p,
+                   const double* W_lut, const double* Ws_lut, int* status, void* stream);
+
+/* Same, fused with an inverse FFT: `spec` holds the spectral rows of the field
+ * (Nr x Nt complex128); the time-domain field is formed in LDS and never
+ * written. Used by the z-step pipeline (Integrator.propagate,
+ * D3/sim/integrator.py:157). If u_out is non-NULL the time-domain field is
+ * also stored there. */
+int fp_plasma_scan_spec(const double* spec, double* rho, double* rho_s, double* u_out,
+                        const fp_plasma_params* p, const double* W_lut, const double* Ws_lut,
+                        const double* tw, int* status, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched FFT along t of Nr rows of Nt complex128 (numpy.fft.fft / ifft along
+ * axis=1, D3/sim/operators.py:49,52). inverse=1 applies the 1/Nt scale.
+ * in may equal out. */
+int fp_fft_rows(const double* in, double* out, int Nr, int Nt, int inverse, const double* tw,
+                void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Hankel GEMM on FP64 MFMA: C = diag(rowscale) * alpha * (Y @ B), optionally
+ * followed by the complex multiply C[m, f] *= phase[m, f].
+ * Replaces cp.dot(self.Y, u) (D3/sim/operators.py:48,53).
+ * Yt: Y transposed (Nr x Nr, row-major Yt[k][m] = Y[m][k]).
+ * B, C: Nr x Nt complex128 viewed as Nr x 2Nt real. rowscale (Nr) and phase
+ * (Nr x Nt complex128) may be NULL. C must not alias B. */
+int fp_hankel_gemm(const double* Yt, const double* B, double* C, int Nr, int Nt, double alpha,
+                   const double* rowscale, const double* phase, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * LinearOperator.half_linear (D3/sim/operators.py:47-53) on time-domain rows:
+ * out = iRk * Y @ ifft(phase_half * fft(Rk * Y @ u)), with
+ * phase_half[m, f] = Rk*iRk... folded: phase_half = exp(i phi dz/2) * spec_mask.
+ * work: 2 x Nr x Nt complex128 scratch. u may equal out. */
+int fp_half_linear(const double* u, double* out, int Nr, int Nt, const double* Yt,
+                   const double* phase_half, double Rk, double iRk, const double* tw,
+                   double* work, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * NonlinearOperator.split (D3/sim/operators.py:89-146): rhs (Nr x Nt
+ * complex128) and alpha (Nr x Nt float64, may be NULL) for a time-domain field.
+ * rho_s may be NULL (no STE index term). */
+int fp_nl_split(const double* u, const double* rho, const double* rho_s, double* rhs,
+                double* alpha, const fp_nl_params* p, const fp_nl_tables* t, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Nonlinear part of Integrator.step (D3/sim/integrator.py:141-147) fused per
+ * row: alpha half-step, classical RK4 over split() with rho frozen, alpha
+ * half-step. in_spectral / out_spectral select spectral or time-domain rows
+ * for input and output (the z-step pipeline uses spectral in and out).
+ * in may equal out. */
+int fp_nl_step(const double* in, double* out, const double* rho, const double* rho_s,
+               int in_spectral, int out_spectral, const fp_nl_params* p, const fp_nl_tables* t,
+               void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Record reductions of Integrator._record / NonlinearOperator.loss_rates
+ * (D3/sim/integrator.py:182-212; D3/sim/operators.py:148-163) for the
+ * time-domain field u and densities rho, rho_s (rho_s may be NULL).
+ * rows_out: 7 x Nr float64 = [sum_t |u|^2, max_t rho, max_t rho_s,
+ *   max_t I, sum_t dep, sum_t 2 photo I, sum_t 2 plasma I] where
+ * I = |u|^2 invE2 and dep = 2 (photo + plasma) I dt 100.
+ * bins (optional, nbins x Nr): sum_t dep over t-index ranges
+ * [bin_lo[b], bin_hi[b]). */
+int fp_record_rows(const double* u, const double* rho, const double* rho_s, double* rows_out,
+                   const fp_nl_params* p, const fp_nl_tables* t, double dt, int nbins,
+                   const int* bin_lo, const int* bin_hi, double* bins, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAPPORT_AMD_H */

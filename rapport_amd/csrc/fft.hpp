@@ -1,0 +1,137 @@
+// LDS-resident Stockham FFT along t for one row held by one workgroup.
+//
+// Layout: a workgroup of TPB threads owns one row of N complex samples; thread
+// `tid` keeps E = N/TPB samples in registers in the *stride layout*
+// v[s] = x[tid + s*TPB]. Each radix-R pass does E/R butterflies per thread on
+// its own registers (butterfly j = tid + q*TPB reads x[j + r*N/R], which is
+// register q + r*E/R), writes the Stockham output positions to LDS, and the
+// next pass reads the stride layout back. The last pass writes x[j + r*N/R]
+// again, so input and output share the stride layout and a row can go
+// forward -> pointwise multiply -> inverse without leaving registers.
+//
+// Conventions are numpy's: forward X[k] = sum_n x[n] exp(-2 pi i k n / N),
+// inverse has +i and is NOT scaled here (callers fold the 1/N).
+//
+// Twiddles come from a per-N table tw[m] = exp(-2 pi i m / N), m < N.
+#pragma once
+#include "common.hpp"
+
+namespace fpfft {
+
+FP_DEV constexpr int lds_pad(int i) { return i + (i >> 4); }
+template <int N> struct LdsSize { static constexpr int value = N + N / 16; };
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+constexpr int bitrev(int r, int nbits) {
+  int o = 0;
+  for (int b = 0; b < nbits; ++b) o |= ((r >> b) & 1) << (nbits - 1 - b);
+  return o;
+}
+
+// cos / sin of 2*pi*k/16
+template <typename T> FP_DEV T c16(int k) {
+  constexpr double tab[8] = {1.0, 0.92387953251128675613, 0.70710678118654752440,
+                             0.38268343236508977173, 0.0, -0.38268343236508977173,
+                             -0.70710678118654752440, -0.92387953251128675613};
+  return T(tab[k & 7]);
+}
+template <typename T> FP_DEV T s16(int k) {
+  constexpr double tab[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440,
+                             0.92387953251128675613, 1.0, 0.92387953251128675613,
+                             0.70710678118654752440, 0.38268343236508977173};
+  return T(tab[k & 7]);
+}
+
+// x * exp(-+2 pi i idx/16), idx in [0, 8)
+template <typename T, bool INV> FP_DEV cpx<T> rot16(cpx<T> x, int idx) {
+  if (idx == 0) return x;
+  if (idx == 4) return mul_mi<T, INV>(x);
+  const T c = c16<T>(idx), s = INV ? s16<T>(idx) : -s16<T>(idx);
+  return mk<T>(x.x * c - x.y * s, x.x * s + x.y * c);
+}
+
+// In-register DFT of size R (R in {2,4,8,16}), natural order in and out.
+template <typename T, int R, bool INV> FP_DEV void dft(cpx<T> (&a)[R]) {
+  constexpr int LB = ilog2(R);
+#pragma unroll
+  for (int len = R; len >= 2; len >>= 1) {
+    const int half = len >> 1;
+#pragma unroll
+    for (int st = 0; st < R; st += len) {
+#pragma unroll
+      for (int m = 0; m < half; ++m) {
+        cpx<T> x = a[st + m], y = a[st + m + half];
+        a[st + m] = x + y;
+        a[st + m + half] = rot16<T, INV>(x - y, m * (R / len) * (16 / R));
+      }
+    }
+  }
+  cpx<T> b[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) b[r] = a[bitrev(r, LB)];
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[r] = b[r];
+}
+
+template <typename T, int N, int TPB, bool INV, int Ns>
+FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw, int tid) {
+  constexpr int E = N / TPB;
+  constexpr int REM = N / Ns;
+  constexpr int R = REM >= E ? E : REM;
+  constexpr bool LAST = (Ns * R == N);
+  constexpr int Q = E / R;
+  constexpr int STRIDE = N / (Ns * R);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = tid + q * TPB;
+    const int k = j & (Ns - 1);
+    cpx<T> a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
+    if (Ns > 1) {
+      if (sizeof(T) == 8) {
+        cpx<T> w1 = tw[k * STRIDE];
+        if (INV) w1 = conjx(w1);
+        cpx<T> w = w1;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          a[r] = a[r] * w;
+          if (r + 1 < R) w = w * w1;
+        }
+      } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cpx<T> w = tw[k * r * STRIDE];
+          if (INV) w = conjx(w);
+          a[r] = a[r] * w;
+        }
+      }
+    }
+    dft<T, R, INV>(a);
+    if (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * Q] = a[r];
+    } else {
+      const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[lds_pad(idxD + r * Ns)] = a[r];
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = lds[lds_pad(tid + s * TPB)];
+    __syncthreads();
+    passes<T, N, TPB, INV, Ns * R>(v, lds, tw, tid);
+  }
+}
+
+// Full-row FFT in the stride layout. `lds` must hold LdsSize<N>::value entries
+// and must not be in use by other threads of the workgroup when called.
+template <typename T, int N, int TPB, bool INV>
+FP_DEV void row_fft(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw) {
+  static_assert(N % TPB == 0, "N must be a multiple of TPB");
+  passes<T, N, TPB, INV, 1>(v, lds, tw, (int)threadIdx.x);
+}
+
+}  // namespace fpfft

@@ -1,0 +1,132 @@
+// Hankel transform as a real x complex GEMM on FP64 MFMA (gfx950).
+//
+// C (M x Ncols) = alpha * diag(rowscale) * Y (M x K) @ B (K x Ncols), where B
+// and C are complex rows viewed as real (Ncols = 2 Nt, columns alternate re/im)
+// and Y is real, so one real GEMM does the whole complex product. Optionally
+// the epilogue multiplies each complex output (m, f) by phase[m, f].
+//
+// Tiling: 256 threads = 4 waves per 128 x 128 tile of C; each wave owns a
+// 64 x 64 quadrant as 4 x 4 blocks of v_mfma_f64_16x16x4f64. K advances in
+// slabs of 16 staged through LDS with double buffering (global -> registers
+// for slab s+1 while MFMAs run on slab s). A is read from Y^T (K x M) so both
+// LDS tiles are k-major and every MFMA operand read is a contiguous 16-lane
+// row. Block indices are remapped so that each XCD works on a contiguous run
+// of column panels (all row tiles of a panel land on one XCD's L2).
+#pragma once
+#include "common.hpp"
+
+namespace fpgemm {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 16, NTHR = 256;
+constexpr int APAD = 4;  // LDS row padding (doubles)
+
+__global__ __launch_bounds__(NTHR) void hankel_gemm_f64(const double* __restrict__ Yt,
+                                                         const double* __restrict__ B,
+                                                         double* __restrict__ C, int M, int K,
+                                                         int Ncols, double alpha,
+                                                         const double* __restrict__ rowscale,
+                                                         const double* __restrict__ phase) {
+  __shared__ double As[2][BK][BM + APAD];
+  __shared__ double Bs[2][BK][BN + APAD];
+
+  const int tilesM = (M + BM - 1) / BM;
+  const int tilesN = (Ncols + BN - 1) / BN;
+  const int nb = tilesM * tilesN;
+  // XCD-aware remap: hardware places block id on XCD (id % 8).
+  int id = blockIdx.x;
+  int lin = id;
+  if (nb % 8 == 0) lin = (id % 8) * (nb / 8) + id / 8;
+  const int tm = lin % tilesM, tn = lin / tilesM;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+
+  // global->LDS staging: each thread moves 8 doubles of A and 8 of B per slab.
+  // A slab: rows k0..k0+15 of Yt, cols m0..m0+127 -> thread: k = tid/16, 8 cols at (tid%16)*8
+  const int ldk = tid >> 4;
+  const int ldc = (tid & 15) * 8;
+  double ra[8], rb[8];
+
+  auto gload = [&](int k0) {
+    const int k = k0 + ldk;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int m = m0 + ldc + i, n = n0 + ldc + i;
+      ra[i] = (k < K && m < M) ? Yt[(size_t)k * M + m] : 0.0;
+      rb[i] = (k < K && n < Ncols) ? B[(size_t)k * Ncols + n] : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      As[buf][ldk][ldc + i] = ra[i];
+      Bs[buf][ldk][ldc + i] = rb[i];
+    }
+  };
+
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int nslab = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int li = lane & 15, lk = lane >> 4;
+  for (int s = 0; s < nslab; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nslab) gload((s + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[cur][kk + lk][wm + a * 16 + li];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[cur][kk + lk][wn + b * 16 + li];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    if (s + 1 < nslab) {
+      sstore(cur ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // epilogue: for v_mfma_f64_16x16x4f64 lane l, register v holds D[l/16 + 4 v][l % 16]
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + wm + a * 16 + lk + 4 * v;
+      const double rs = (rowscale && m < M) ? alpha * rowscale[m] : alpha;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int n = n0 + wn + b * 16 + li;
+        double val = acc[a][b][v] * rs;
+        if (phase) {
+          const double other = __shfl_xor(val, 1, 64);
+          const int f = n >> 1;
+          double pr = 0.0, pi = 0.0;
+          if (m < M && n < Ncols) {
+            pr = phase[((size_t)m * (Ncols >> 1) + f) * 2];
+            pi = phase[((size_t)m * (Ncols >> 1) + f) * 2 + 1];
+          }
+          // even lane holds re, odd lane holds im of the same (m, f)
+          val = (n & 1) ? (other * pi + val * pr) : (val * pr - other * pi);
+        }
+        if (m < M && n < Ncols) C[(size_t)m * Ncols + n] = val;
+      }
+    }
+  }
+}
+
+}  // namespace fpgemm

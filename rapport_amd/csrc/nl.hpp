@@ -1,0 +1,368 @@
+// Nonlinear operator of the split step, one workgroup per radial row.
+//
+// split() (D3/sim/operators.py:89-146) evaluated on a row held in registers in
+// the FFT stride layout (fft.hpp): pointwise Keldysh/plasma terms, the Raman
+// convolution and the three spectrally weighted transforms all happen in LDS
+// without touching HBM. The fused step kernel chains the alpha half-steps and
+// the four RK4 stages of Integrator.step (D3/sim/integrator.py:141-147) on the
+// same row, so the whole nonlinear part of a z-step reads the row once and
+// writes it once.
+#pragma once
+#include "common.hpp"
+#include "fft.hpp"
+#include "../../include/rapport_amd.h"
+
+namespace fpnl {
+
+using fpfft::LdsSize;
+using fpfft::row_fft;
+
+// Exact decade boundaries of the PCHIP knot grid 10^k (1 + 0.01 j).
+FP_DEV double p10(int k) {
+  constexpr double t[18] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8,
+                            1e9, 1e10, 1e11, 1e12, 1e13, 1e14, 1e15, 1e16, 1e17};
+  return t[k];
+}
+
+// Interval i with knots[i] <= x < knots[i+1] (x in [knots[0], knots[n-1]];
+// x == knots[n-1] maps to n-2, as scipy.interpolate.PPoly does).
+FP_DEV int pchip_interval(double x, const double* __restrict__ knots, int n) {
+  int k = (int)log10(x);
+  if (k < 0) k = 0;
+  if (k > 16) k = 16;
+  while (k < 16 && x >= p10(k + 1)) ++k;
+  while (k > 0 && x < p10(k)) --k;
+  int j = (int)((x / p10(k) - 1.0) * 100.0);
+  if (j < 0) j = 0;
+  if (j > 899) j = 899;
+  int i = k * 900 + j;
+  if (i > n - 2) i = n - 2;
+  while (i < n - 2 && knots[i + 1] <= x) ++i;
+  while (i > 0 && knots[i] > x) --i;
+  return i;
+}
+
+// |PCHIP(I)| with nan_to_num, FP64 power basis in dx = x - x_i.
+FP_DEV double keldysh_W(double absu2, const fp_nl_params& p, const fp_nl_tables& t) {
+  double x = absu2 * p.invE2;
+  x = fmin(fmax(x, p.Imin), p.Imax);   // cp.clip
+  if (!fp_isfinite(x)) x = p.Imax;     // clip of +inf; NaN handled below
+  const int n = p.n_knots;
+  const int i = pchip_interval(x, t.knots, n);
+  const double dx = x - t.knots[i];
+  const int m = n - 1;
+  const double c0 = t.pcoef[i], c1 = t.pcoef[m + i], c2 = t.pcoef[2 * m + i], c3 = t.pcoef[3 * m + i];
+  const double w = fabs(((c0 * dx + c1) * dx + c2) * dx + c3);
+  return nan0(w);
+}
+
+// photo and alpha of split() (operators.py:91-107) for one sample.
+FP_DEV void photo_alpha(double absu2, double rho, const fp_nl_params& p, const fp_nl_tables& t,
+                        double& photo, double& alpha) {
+  double ph = 0.0;
+  if (p.en_pi_loss) {
+    const double W = keldysh_W(absu2, p, t);
+    double depl = 1.0 - rho / p.rho_max;
+    depl = fmin(fmax(depl, 0.0), 1.0);
+    ph = W * p.photo_pref / (absu2 + 1e-30) * depl;
+  }
+  double a = ph;
+  if (p.en_plasma_absorb) a = a + p.plasma_pref * rho;
+  a = nan0(a);
+  photo = ph;
+  alpha = a > 0.0 ? a : 0.0;
+}
+
+// One split() evaluation on a row in registers: k = rhs(us), al = alpha(us).
+template <int NT, int TPB, bool RS>
+FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double (&rho)[NT / TPB],
+                      const double (&rhos)[RS ? NT / TPB : 1], cpx<double> (&k)[NT / TPB],
+                      double (&al)[NT / TPB], const fp_nl_params& p, const fp_nl_tables& t,
+                      cpx<double>* lds) {
+  constexpr int E = NT / TPB;
+  const int tid = threadIdx.x;
+  const cpx<double>* tw = reinterpret_cast<const cpx<double>*>(t.tw);
+  const cpx<double>* Rf = reinterpret_cast<const cpx<double>*>(t.Rf);
+  const double invN = 1.0 / (double)NT;
+  double absu2[E];
+  unsigned zmask = 0;
+  cpx<double> buf[E], nl[E];
+  // pointwise terms; B = photo * u goes straight into the transform buffer
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
+    absu2[s] = a2;
+    if (!(a2 >= 1e-30)) zmask |= 1u << s;   // where(absu2 < 1e-30) (NaN -> nan_to_num 0 anyway)
+    double ph, a;
+    photo_alpha(a2, rho[s], p, t, ph, a);
+    al[s] = a;
+    buf[s] = ph * us[s];
+  }
+  const bool any_photo = p.en_pi_loss;
+  if (any_photo) {
+    row_fft<double, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) nl[s] = t.mB[tid + s * TPB] * buf[s];
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) nl[s] = mk<double>(0.0, 0.0);
+  }
+  // Kerr intensity with the Raman convolution
+  double kerr[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) kerr[s] = p.en_kerr_inst ? (1.0 - p.f_R) * absu2[s] : 0.0;
+  if (p.en_kerr_raman) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) buf[s] = mk<double>(absu2[s], 0.0);
+    row_fft<double, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) buf[s] = buf[s] * Rf[tid + s * TPB];
+    row_fft<double, NT, TPB, true>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) kerr[s] = kerr[s] + p.f_R * (buf[s].x * invN);
+  }
+  if (p.en_kerr_inst || p.en_kerr_raman) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const double kk = p.kerr_pref * kerr[s];
+      buf[s] = mk<double>(-kk * us[s].y, kk * us[s].x);   // i * kappa * kerr_I * u
+    }
+    row_fft<double, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) nl[s] = nl[s] + t.mA[tid + s * TPB] * buf[s];
+  }
+  // extra = -pp * pc * rho * u + i * ste * rho_s * u
+  const bool any_extra = (p.pc_re != 0.0 || p.pc_im != 0.0) || (RS && p.ste_pref != 0.0);
+  if (any_extra) {
+    const cpx<double> g = mk<double>(-p.plasma_pref * p.pc_re, -p.plasma_pref * p.pc_im);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      cpx<double> e = rho[s] * (g * us[s]);
+      if (RS && p.ste_pref != 0.0) {
+        const double q = p.ste_pref * rhos[s];
+        e = e + mk<double>(-q * us[s].y, q * us[s].x);
+      }
+      buf[s] = e;
+    }
+    row_fft<double, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) nl[s] = nl[s] + t.mC[tid + s * TPB] * buf[s];
+  }
+  row_fft<double, NT, TPB, true>(nl, lds, tw);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    cpx<double> r = mk<double>(nl[s].x * invN + al[s] * us[s].x, nl[s].y * invN + al[s] * us[s].y);
+    if (zmask & (1u << s)) r = mk<double>(0.0, 0.0);
+    k[s] = mk<double>(nan0(r.x), nan0(r.y));
+  }
+}
+
+template <int NT, int TPB>
+FP_DEV void load_row(const double* __restrict__ src, size_t row, cpx<double> (&v)[NT / TPB]) {
+  const cpx<double>* r = reinterpret_cast<const cpx<double>*>(src) + row * NT;
+#pragma unroll
+  for (int s = 0; s < NT / TPB; ++s) v[s] = r[threadIdx.x + s * TPB];
+}
+template <int NT, int TPB>
+FP_DEV void store_row(double* __restrict__ dst, size_t row, const cpx<double> (&v)[NT / TPB]) {
+  cpx<double>* r = reinterpret_cast<cpx<double>*>(dst) + row * NT;
+#pragma unroll
+  for (int s = 0; s < NT / TPB; ++s) r[threadIdx.x + s * TPB] = v[s];
+}
+
+// split(): rhs and alpha of a time-domain field.
+template <int NT, int TPB, bool RS>
+__global__ __launch_bounds__(TPB) void split_kernel(const double* __restrict__ u,
+                                                     const double* __restrict__ rho,
+                                                     const double* __restrict__ rhos,
+                                                     double* __restrict__ rhs,
+                                                     double* __restrict__ alpha, fp_nl_params p,
+                                                     fp_nl_tables t) {
+  __shared__ cpx<double> lds[LdsSize<NT>::value];
+  constexpr int E = NT / TPB;
+  const size_t row = blockIdx.x;
+  cpx<double> us[E], k[E];
+  double r[E], rs[RS ? E : 1], al[E];
+  load_row<NT, TPB>(u, row, us);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const size_t i = row * NT + threadIdx.x + s * TPB;
+    r[s] = rho[i];
+    if (RS) rs[s] = rhos[i];
+  }
+  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
+  store_row<NT, TPB>(rhs, row, k);
+  if (alpha) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) alpha[row * NT + threadIdx.x + s * TPB] = al[s];
+  }
+}
+
+// alpha half-step: u *= exp(-0.5 dz alpha(u, rho))
+template <int E>
+FP_DEV void alpha_half(cpx<double> (&u)[E], const double (&rho)[E], const fp_nl_params& p,
+                       const fp_nl_tables& t) {
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const double a2 = u[s].x * u[s].x + u[s].y * u[s].y;
+    double ph, a;
+    photo_alpha(a2, rho[s], p, t, ph, a);
+    const double f = exp(-0.5 * p.dz * a);
+    u[s] = f * u[s];
+  }
+}
+
+// Nonlinear part of Integrator.step fused over one row. The RK4 base field is
+// parked in LDS (stride layout) to keep the register state of a stage small.
+template <int NT, int TPB, bool RS>
+__global__ __launch_bounds__(TPB) void nl_step_kernel(const double* in, double* out,
+                                                       const double* __restrict__ rho,
+                                                       const double* __restrict__ rhos,
+                                                       int in_spectral, int out_spectral,
+                                                       fp_nl_params p, fp_nl_tables t) {
+  __shared__ cpx<double> lds[LdsSize<NT>::value];
+  __shared__ cpx<double> ub[NT];
+  constexpr int E = NT / TPB;
+  const int tid = threadIdx.x;
+  const size_t row = blockIdx.x;
+  const cpx<double>* tw = reinterpret_cast<const cpx<double>*>(t.tw);
+  cpx<double> us[E];
+  load_row<NT, TPB>(in, row, us);
+  if (in_spectral) {
+    row_fft<double, NT, TPB, true>(us, lds, tw);
+    const double invN = 1.0 / (double)NT;
+#pragma unroll
+    for (int s = 0; s < E; ++s) us[s] = invN * us[s];
+  }
+  double r[E], rs[RS ? E : 1];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const size_t i = row * NT + tid + s * TPB;
+    r[s] = rho[i];
+    if (RS) rs[s] = rhos[i];
+  }
+  alpha_half<E>(us, r, p, t);
+#pragma unroll
+  for (int s = 0; s < E; ++s) ub[tid + s * TPB] = us[s];
+  const double dz = p.dz;
+  cpx<double> acc[E], k[E];
+  double al[E];
+  // k1 (us == u)
+  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    acc[s] = k[s];
+    us[s] = ub[tid + s * TPB] + (0.5 * dz) * k[s];
+  }
+  // k2
+  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    acc[s] = acc[s] + 2.0 * k[s];
+    us[s] = ub[tid + s * TPB] + (0.5 * dz) * k[s];
+  }
+  // k3
+  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    acc[s] = acc[s] + 2.0 * k[s];
+    us[s] = ub[tid + s * TPB] + dz * k[s];
+  }
+  // k4
+  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    acc[s] = acc[s] + k[s];
+    us[s] = ub[tid + s * TPB] + (dz / 6.0) * acc[s];
+  }
+  alpha_half<E>(us, r, p, t);
+  if (out_spectral) row_fft<double, NT, TPB, false>(us, lds, tw);
+  store_row<NT, TPB>(out, row, us);
+}
+
+// Record reductions (integrator.py:182-212, operators.py:148-163), one row per block.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void record_kernel(const double* __restrict__ u,
+                                                      const double* __restrict__ rho,
+                                                      const double* __restrict__ rhos,
+                                                      double* __restrict__ rows_out, fp_nl_params p,
+                                                      fp_nl_tables t, double dt, int nbins,
+                                                      const int* __restrict__ bin_lo,
+                                                      const int* __restrict__ bin_hi,
+                                                      double* __restrict__ bins) {
+  constexpr int NV = 7;
+  __shared__ double red[NV][TPB / 64];
+  __shared__ double bred[TPB / 64];
+  const int Nt = p.Nt, Nr = p.Nr;
+  const size_t row = blockIdx.x;
+  const cpx<double>* ur = reinterpret_cast<const cpx<double>*>(u) + row * Nt;
+  double v[NV] = {0.0, 0.0, -INFINITY, -INFINITY, 0.0, 0.0, 0.0};
+  v[1] = -INFINITY;
+  for (int i = threadIdx.x; i < Nt; i += TPB) {
+    const cpx<double> z = ur[i];
+    const double a2 = z.x * z.x + z.y * z.y;
+    const double rr = rho[row * Nt + i];
+    const double rs = rhos ? rhos[row * Nt + i] : 0.0;
+    const double I = a2 * p.invE2;
+    double ph = 0.0;
+    if (p.en_pi_loss) {
+      const double W = keldysh_W(a2, p, t);
+      double depl = fmin(fmax(1.0 - rr / p.rho_max, 0.0), 1.0);
+      ph = W * p.photo_pref / (a2 + 1e-30) * depl;
+    }
+    const double pl = p.en_plasma_absorb ? p.plasma_pref * rr : 0.0;
+    v[0] += a2;
+    v[1] = fmax(v[1], rr);
+    v[2] = fmax(v[2], rs);
+    v[3] = fmax(v[3], I);
+    v[4] += 2.0 * (ph + pl) * I * dt * 100.0;
+    v[5] += 2.0 * ph * I;
+    v[6] += 2.0 * pl * I;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = v[q];
+    for (int d = 32; d >= 1; d >>= 1) {
+      const double o = __shfl_xor(x, d, 64);
+      x = (q >= 1 && q <= 3) ? fmax(x, o) : x + o;
+    }
+    if (lane == 0) red[q][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int q = threadIdx.x;
+    double x = red[q][0];
+    for (int w = 1; w < TPB / 64; ++w) x = (q >= 1 && q <= 3) ? fmax(x, red[q][w]) : x + red[q][w];
+    rows_out[(size_t)q * Nr + row] = x;
+  }
+  for (int b = 0; b < nbins; ++b) {
+    double x = 0.0;
+    for (int i = threadIdx.x; i < Nt; i += TPB) {
+      if (i < bin_lo[b] || i >= bin_hi[b]) continue;
+      const cpx<double> z = ur[i];
+      const double a2 = z.x * z.x + z.y * z.y;
+      const double rr = rho[row * Nt + i];
+      const double I = a2 * p.invE2;
+      double ph = 0.0;
+      if (p.en_pi_loss) {
+        const double W = keldysh_W(a2, p, t);
+        double depl = fmin(fmax(1.0 - rr / p.rho_max, 0.0), 1.0);
+        ph = W * p.photo_pref / (a2 + 1e-30) * depl;
+      }
+      const double pl = p.en_plasma_absorb ? p.plasma_pref * rr : 0.0;
+      x += 2.0 * (ph + pl) * I * dt * 100.0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    __syncthreads();
+    if (lane == 0) bred[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < TPB / 64; ++w) s += bred[w];
+      bins[(size_t)b * Nr + row] = s;
+    }
+  }
+}
+
+}  // namespace fpnl

@@ -1,0 +1,366 @@
+"""Z-marching driver: drop-in for ``Integrator`` of ``D3/sim/integrator.py:28-340``
+(D3 = ``.github/deliverable (3)``).
+
+Same loop semantics as the reference: ``propagate()`` runs ``nz + 1`` iterations;
+iteration i recomputes the carrier densities from the current field (frozen
+for the whole step), applies the Strang step
+``L/2 -> alpha/2 -> RK4(N) -> alpha/2 -> L/2 -> x mask_r`` and records the state
+with the label ``z = begin + i dz``. Outputs (``result.npz`` keys and
+``params.json``) follow integrator.py:238-340.
+
+GPU pipeline of one iteration (all on one stream, no host synchronisation):
+
+    plasma scan   spectral field -> IFFT_t (LDS) -> rho, rho_s          [row kernel]
+    L/2           G1: Rk Y @ . x phase   G2: iRk Y @ .                   [FP64 MFMA GEMMs]
+    N             IFFT_t -> alpha/2 -> RK4 (4 x split) -> alpha/2 -> FFT_t [row kernel]
+    L/2 + mask    G3: Rk Y @ . x phase   G4: iRk mask_r Y @ .
+
+The field is kept between iterations as its FFT along t (the Hankel matrix acts
+along r and commutes with the FFT along t, and FFT(IFFT(x)) = x), so the only
+time-domain copies formed are inside the row kernels; ``Integrator.u`` converts
+on demand. Recording, when due, adds one inverse FFT and one reduction kernel
+and writes into device-resident histories.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from typing import Callable, Union
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_ptr
+from .config import Config, code_fingerprint
+from .grids import ENVELOPES, build_grids
+from .operators import DeviceTables, LinearOperator, NonlinearOperator
+
+
+class Integrator:
+    def __init__(self, cfg: Config, envelope: Union[str, Callable] = "gaussian_focused",
+                 device=None):
+        self.cfg = cfg
+        if cfg.precision != "fp64":
+            from .mixed import MixedIntegrator  # noqa: F401  (raises if unavailable)
+        if cfg.write_output:
+            os.makedirs(cfg.out_dir, exist_ok=True)
+        g = build_grids(cfg)
+        self.g = g
+        self.dz, self.dt = cfg.dz, g["dt"]
+        self.t = DeviceTables(cfg, g, device)
+        dev = self.t.device
+        self.device = dev
+        self.lin = LinearOperator(cfg, g, self.t)
+        self.nl = NonlinearOperator(cfg, g, self.t)
+
+        fn = envelope if callable(envelope) else ENVELOPES[envelope]
+        u0 = fn(g["rr"], g["tt"], cfg, g)
+        if isinstance(u0, torch.Tensor):
+            u0 = u0.detach().cpu().numpy()
+        u0 = np.ascontiguousarray(np.asarray(u0).astype(np.complex128, copy=False))
+        self.Nr, self.Nt = u0.shape
+        self._u = torch.from_numpy(u0).to(dev)
+        self._spec = None                       # FFT_t of the field when the pipeline runs
+        self.rho = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
+        self.rho_s = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
+        self.mask_r = self.t.mask_r
+
+        r = g["rlist"]
+        self._r_w = 2.0 * np.pi * r * np.diff(r, prepend=0.0)   # 2 pi r dr (rectangle)
+        fl0 = np.sum(np.abs(u0) ** 2, axis=1) * self.dt * g["invE2"] * 1e4
+        self.U0_uJ = float(np.sum(fl0 * self._r_w)) * 1e6
+        if cfg.verbose:
+            print(f"[init] U_beam(0) = {self.U0_uJ:.3f} uJ ", flush=True)
+
+        n_saves = cfg.nz // cfg.save_stride + 1
+        self.n_saves = n_saves
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.fluence_rz = torch.zeros(n_saves, self.Nr, **f32)
+        self.rho_rz = torch.zeros(n_saves, self.Nr, **f32)
+        self.rho_s_rz = torch.zeros(n_saves, self.Nr, **f32)
+        self.Imax_z = np.zeros(n_saves)
+        self._Imax_dev = torch.zeros(n_saves, dtype=torch.float64, device=dev)
+        self.z_saved = np.zeros(n_saves)
+        self._dEdz_dev = torch.zeros(2, n_saves, dtype=torch.float64, device=dev)
+        self._dEdz_photo_uJm = np.zeros(n_saves)
+        self._dEdz_plasma_uJm = np.zeros(n_saves)
+        self.E_STE_z = np.zeros(n_saves)
+        self.k_save = 0
+        self.t_full_fs = g["tlist"] * 1e15
+
+        self._t_stride = max(1, cfg.rho_t_stride) if cfg.rho_t_stride > 0 else 0
+        self._r_stride = max(1, cfg.rho_r_stride)
+        if self._t_stride > 0:
+            Nt_sub = (self.Nt - 1) // self._t_stride + 1
+            Nr_sub = (self.Nr - 1) // self._r_stride + 1
+            self.rho_rzt = torch.zeros(n_saves, Nr_sub, Nt_sub, **f32)
+            self.rho_s_rzt = torch.zeros(n_saves, Nr_sub, Nt_sub, **f32)
+            self.I_rzt = torch.zeros(n_saves, Nr_sub, Nt_sub, **f32)
+            self.t_sub_fs = self.t_full_fs[::self._t_stride]
+            self.r_sub = r[::self._r_stride]
+        else:
+            self.rho_rzt = self.rho_s_rzt = self.I_rzt = None
+            self.t_sub_fs = self.r_sub = None
+        cube_b = 3 * 4 * self.rho_rzt.numel() if self.rho_rzt is not None else 0
+        rest_b = 4 * n_saves * (10 * 2 * self.Nr + 3 * self.Nt)
+        self.size_estimate_gb = (cube_b + rest_b) / 1024 ** 3
+        if cfg.verbose:
+            print(f"[init] result.npz ~ {self.size_estimate_gb:.2f} GB non compresse"
+                  + (f" ({100 * cube_b / (cube_b + rest_b):.0f} % = cube (z,r,t))" if cube_b else ""),
+                  flush=True)
+        if self.size_estimate_gb > 2.0:
+            print(f"[WARN] result.npz estimated at {self.size_estimate_gb:.1f} GB; the (z,r,t) cube is "
+                  f"set by rho_t_stride={cfg.rho_t_stride} / rho_r_stride={cfg.rho_r_stride}; "
+                  "rho_t_stride=0 disables it.", flush=True)
+
+        self.absorbed_rz = torch.zeros(n_saves, self.Nr, **f32)
+        tl = self.t_full_fs
+        if cfg.absorb_time_bins_fs:
+            edges = np.asarray(cfg.absorb_time_bins_fs, dtype=np.float64)
+            lo, hi = [], []
+            for a, b in zip(edges[:-1], edges[1:]):
+                idx = np.nonzero((tl >= a) & (tl < b))[0]   # tlist increases: a contiguous range
+                lo.append(int(idx[0]) if idx.size else 0)
+                hi.append(int(idx[-1]) + 1 if idx.size else 0)
+            self._bin_lo = torch.tensor(lo, dtype=torch.int32, device=dev)
+            self._bin_hi = torch.tensor(hi, dtype=torch.int32, device=dev)
+            self.absorb_bin_edges_fs = edges
+            self.absorbed_rz_bins = torch.zeros(n_saves, len(lo), self.Nr, **f32)
+            self._bins_tmp = torch.zeros(len(lo), self.Nr, dtype=torch.float64, device=dev)
+        else:
+            self._bin_lo = self._bin_hi = None
+            self.absorb_bin_edges_fs = self.absorbed_rz_bins = None
+        self.Ipeak_rz = torch.zeros(n_saves, self.Nr, **f32)
+        if cfg.rho_snapshot_t_fs is not None:
+            self._it_snap = int(np.argmin(np.abs(tl - cfg.rho_snapshot_t_fs)))
+            self.rho_rz_at = torch.zeros(n_saves, self.Nr, **f32)
+        else:
+            self._it_snap, self.rho_rz_at = None, None
+        self.rho_onaxis_t = torch.zeros(n_saves, self.Nt, **f32)
+        self.rho_s_onaxis_t = torch.zeros(n_saves, self.Nt, **f32)
+        self.I_onaxis_t = torch.zeros(n_saves, self.Nt, **f32)
+
+        self._rows = torch.zeros(7, self.Nr, dtype=torch.float64, device=dev)
+        self._r_w_dev = torch.from_numpy(self._r_w).to(dev)
+        self._w0 = self._w1 = None
+        self._utmp = None
+
+    # ------------------------------------------------------------------ state
+    @property
+    def u(self):
+        """Time-domain field (Nr x Nt complex128 on the GPU)."""
+        if self._spec is not None:
+            self._u = torch.empty_like(self._spec)
+            call("fp_fft_rows", ptr(self._spec), ptr(self._u), self.Nr, self.Nt, 1, ptr(self.t.tw),
+                 stream_ptr())
+            self._spec = None
+        return self._u
+
+    @u.setter
+    def u(self, value):
+        self._u = value.contiguous()
+        self._spec = None
+
+    def _spectral(self):
+        if self._spec is None:
+            self._spec = torch.empty_like(self._u)
+            call("fp_fft_rows", ptr(self._u), ptr(self._spec), self.Nr, self.Nt, 0, ptr(self.t.tw),
+                 stream_ptr())
+            self._u = None
+        return self._spec
+
+    def _rho_s_arg(self):
+        return self.rho_s if (self.cfg.enable_ste and self.g["ste_pref"]) else None
+
+    # ------------------------------------------------------------------ one step
+    def step(self, dz):
+        """Integrator.step (integrator.py:138-149) on the time-domain field."""
+        u = self.lin.half_linear(self.u, dz)
+        call("fp_nl_step", ptr(u), ptr(u), ptr(self.rho), ptr(self._rho_s_arg()), 0, 0,
+             ctypes.byref(self.nl.params(dz)), ctypes.byref(self.t.tables), stream_ptr())
+        u = self.lin.half_linear(u, dz)
+        self.u = u * self.mask_r[:, None]
+
+    def _pipeline_step(self, dz):
+        """One propagate() iteration body (plasma + step) on the spectral state."""
+        t = self.t
+        spec = self._spectral()
+        if self._w0 is None:
+            self._w0 = torch.empty_like(spec)
+            self._w1 = torch.empty_like(spec)
+        w0, w1 = self._w0, self._w1
+        st = stream_ptr()
+        pp = t.plasma_params
+        call("fp_plasma_scan_spec", ptr(spec), ptr(self.rho), ptr(self.rho_s), None,
+             ctypes.byref(pp), ptr(t.W_lut), ptr(t.Ws_lut), ptr(t.tw), ptr(t.status), st)
+        phase = t.phase_half(dz)
+        call("fp_hankel_gemm", ptr(t.Yt), ptr(spec), ptr(w1), self.Nr, self.Nt, t.Rk, None,
+             ptr(phase), st)
+        call("fp_hankel_gemm", ptr(t.Yt), ptr(w1), ptr(w0), self.Nr, self.Nt, t.iRk, None, None, st)
+        call("fp_nl_step", ptr(w0), ptr(w0), ptr(self.rho), ptr(self._rho_s_arg()), 1, 1,
+             ctypes.byref(self.nl.params(dz)), ctypes.byref(t.tables), st)
+        call("fp_hankel_gemm", ptr(t.Yt), ptr(w0), ptr(w1), self.Nr, self.Nt, t.Rk, None,
+             ptr(phase), st)
+        call("fp_hankel_gemm", ptr(t.Yt), ptr(w1), ptr(spec), self.Nr, self.Nt, t.iRk,
+             ptr(t.mask_r), None, st)
+
+    def advance(self, i):
+        """Body of the propagate() loop for iteration i (integrator.py:157-162)."""
+        self._pipeline_step(self.dz)
+        if i % self.cfg.save_stride == 0:
+            self._record(self.cfg.begin + i * self.dz)
+
+    def propagate(self, n_steps=None):
+        cfg = self.cfg
+        n = cfg.nz + 1 if n_steps is None else int(n_steps)
+        it = range(n)
+        pbar = None
+        if cfg.verbose:
+            try:
+                from tqdm.auto import tqdm
+                pbar = tqdm(it, desc="Filamentation", unit="step")
+                it = pbar
+            except ImportError:
+                pass
+        for i in it:
+            self.advance(i)
+            if pbar is not None and i > 0 and i % cfg.ckpt_every == 0 and self.k_save > 0:
+                k = self.k_save - 1
+                flu = self.fluence_rz[k].double().cpu().numpy()
+                U = float(np.sum(flu * self._r_w)) * 100.0
+                pbar.set_postfix(z=f"{(cfg.begin + i * self.dz) * 1e6:+.0f}um",
+                                 U=f"{U / self.U0_uJ * 100:.1f}%",
+                                 I_peak=f"{float(self._Imax_dev[k]):.2e}")
+        if pbar is not None:
+            pbar.close()
+        if int(self.t.status.item()) != 0:
+            print(f"[WARN] plasma scan hit its iteration cap on {int(self.t.status.item())} rows")
+        return self._results()
+
+    # ------------------------------------------------------------------ recording
+    def _record(self, z_now):
+        """integrator.py:179-224 with device-side reductions."""
+        k, g, cfg = self.k_save, self.g, self.cfg
+        u = self.u if self._spec is None else self._time_copy()
+        t = self.t
+        nb = 0 if self._bin_lo is None else int(self._bin_lo.numel())
+        call("fp_record_rows", ptr(u), ptr(self.rho), ptr(self.rho_s), ptr(self._rows),
+             ctypes.byref(self.nl.params(self.dz)), ctypes.byref(t.tables), self.dt, nb,
+             ptr(self._bin_lo), ptr(self._bin_hi), ptr(self._bins_tmp) if nb else None,
+             stream_ptr())
+        rows = self._rows
+        invE2, dt = g["invE2"], self.dt
+        self.fluence_rz[k] = (rows[0] * dt * invE2).float()
+        self.rho_rz[k] = rows[1].float()
+        self.rho_s_rz[k] = rows[2].float()
+        self._Imax_dev[k] = rows[3].max()
+        self.z_saved[k] = float(z_now)
+        self.absorbed_rz[k] = rows[4].float()
+        if nb:
+            self.absorbed_rz_bins[k] = self._bins_tmp.float()
+        self.Ipeak_rz[k] = rows[3].float()
+        if self._it_snap is not None:
+            self.rho_rz_at[k] = self.rho[:, self._it_snap].float()
+        self._dEdz_dev[0, k] = (rows[5] * dt * self._r_w_dev).sum() * 1e4 * 1e6
+        self._dEdz_dev[1, k] = (rows[6] * dt * self._r_w_dev).sum() * 1e4 * 1e6
+        if self._t_stride > 0 and self.rho_rzt is not None:
+            rs, ts = self._r_stride, self._t_stride
+            self.rho_rzt[k] = self.rho[::rs, ::ts].float()
+            self.rho_s_rzt[k] = self.rho_s[::rs, ::ts].float()
+            self.I_rzt[k] = ((u[::rs, ::ts].abs() ** 2) * invE2).float()
+        self.rho_onaxis_t[k] = self.rho[0].float()
+        self.rho_s_onaxis_t[k] = self.rho_s[0].float()
+        self.I_onaxis_t[k] = ((u[0].abs() ** 2) * invE2).float()
+        self.k_save += 1
+
+    def _time_copy(self):
+        if self._utmp is None:
+            self._utmp = torch.empty_like(self._spec)
+        call("fp_fft_rows", ptr(self._spec), ptr(self._utmp), self.Nr, self.Nt, 1, ptr(self.t.tw),
+             stream_ptr())
+        return self._utmp
+
+    def _cumulative_energy_fraction(self, dEdz_uJm):
+        """Trapezoid cumulative integral of dE/dz over z, over U0 (integrator.py:226-236)."""
+        z = self.z_saved[:self.k_save]
+        d = dEdz_uJm[:self.k_save]
+        if len(z) < 2:
+            return np.zeros_like(d)
+        seg = 0.5 * (d[:-1] + d[1:]) * np.diff(z)
+        return np.concatenate([[0.0], np.cumsum(seg)]) / self.U0_uJ
+
+    def _results(self):
+        """integrator.py:238-298: mirrored radial axis, cumulative losses, atomic npz."""
+        cfg, g, k = self.cfg, self.g, self.k_save
+        self.Imax_z[:] = self._Imax_dev.cpu().numpy()
+        dE = self._dEdz_dev.cpu().numpy()
+        self._dEdz_photo_uJm[:], self._dEdz_plasma_uJm[:] = dE[0], dE[1]
+        host = lambda x: x[:k].cpu().numpy() if x is not None else None  # noqa: E731
+        mirror = lambda a: np.hstack([a[:, ::-1], a])                     # noqa: E731
+        r = g["rlist"]
+        E_MPI = self._cumulative_energy_fraction(self._dEdz_photo_uJm)
+        E_pl = self._cumulative_energy_fraction(self._dEdz_plasma_uJm)
+        flu, rr, rs = host(self.fluence_rz), host(self.rho_rz), host(self.rho_s_rz)
+        out = dict(
+            r=np.concatenate([-r[::-1], r]), rlist=r, z=self.z_saved[:k],
+            fluence_rz=mirror(flu), rho_rz=mirror(rr), rho_s_rz=mirror(rs),
+            Imax_z=self.Imax_z[:k], E_plasma_z=E_pl, E_MPI_z=E_MPI, E_STE_z=self.E_STE_z[:k],
+            E_total_z=E_MPI + E_pl,
+            rho_rzt=host(self.rho_rzt), rho_s_rzt=host(self.rho_s_rzt), I_rzt=host(self.I_rzt),
+            t_sub_fs=self.t_sub_fs, r_sub=self.r_sub,
+            absorbed_rz=mirror(host(self.absorbed_rz)), Ipeak_rz=mirror(host(self.Ipeak_rz)),
+            rho_rz_at=mirror(host(self.rho_rz_at)) if self.rho_rz_at is not None else None,
+            absorbed_rz_bins=(np.stack([mirror(b) for b in
+                                        np.moveaxis(host(self.absorbed_rz_bins), 1, 0)], axis=1)
+                              if self.absorbed_rz_bins is not None else None),
+            absorb_bin_edges_fs=self.absorb_bin_edges_fs,
+            rho_onaxis_t=host(self.rho_onaxis_t), rho_s_onaxis_t=host(self.rho_s_onaxis_t),
+            I_onaxis_t=host(self.I_onaxis_t), t_full_fs=self.t_full_fs,
+        )
+        if cfg.write_output:
+            final = os.path.join(cfg.out_dir, "result.npz")
+            tmp = final + ".tmp.npz"
+            try:
+                np.savez_compressed(tmp, **out)
+                os.replace(tmp, final)
+            except BaseException:
+                for f in (tmp, tmp + ".npz"):
+                    try:
+                        os.remove(f)
+                    except OSError:
+                        pass
+                raise
+            self._dump_params()
+        return out
+
+    def _dump_params(self):
+        """params.json (integrator.py:300-340)."""
+        cfg = self.cfg
+        params = dict(
+            n0=cfg.n0, n0_probe=cfg.n0_probe, nc_probe_cm3=cfg.nc_probe,
+            lambda_probe_nm=cfg.lambda_probe * 1e9, wavelength_nm=cfg.wavelength * 1e9,
+            n2=cfg.n2, U_g_eV=cfg.Ui_eV, Us_eV=cfg.Us_eV, energy_uJ=cfg.energy_uJ,
+            w0_um=cfg.w0 * 1e6, delta_t_fs=cfg.delta_t * 1e15,
+            code_fingerprint=code_fingerprint(),
+            begin_um=cfg.begin * 1e6, end_um=cfg.end * 1e6, z_focus_air_um=cfg.z_focus_air_um,
+            toggles=dict(
+                enable_kerr_instantaneous=cfg.enable_kerr_instantaneous,
+                enable_kerr_raman=cfg.enable_kerr_raman,
+                enable_self_steepening=cfg.enable_self_steepening,
+                enable_photoionization_loss=cfg.enable_photoionization_loss,
+                enable_plasma_defocusing=cfg.enable_plasma_defocusing,
+                enable_plasma_absorption=cfg.enable_plasma_absorption,
+                enable_space_time_focusing=cfg.enable_space_time_focusing,
+                enable_spectral_filter=cfg.enable_spectral_filter,
+                tau_ste_fs=(cfg.tau_ste * 1e15 if cfg.tau_ste else None),
+                enable_ste_index=cfg.enable_ste_index,
+                E_tr_eV=cfg.E_tr_eV, f_ste_pump=float(self.g["f_ste"]),
+                enable_avalanche=cfg.enable_avalanche,
+                enable_recombination=cfg.enable_recombination,
+                enable_ste=cfg.enable_ste,
+            ),
+        )
+        with open(os.path.join(cfg.out_dir, "params.json"), "w") as f:
+            json.dump(params, f, indent=2)

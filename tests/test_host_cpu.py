@@ -1,0 +1,93 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports
+every declared entry point, and the host setup (Config, Keldysh tables, grids,
+envelopes) equals the oracle's."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import rapport_amd as R
+from oracle import reference_numpy as O
+from tests._cases import C1, C2, cfg_kwargs, envelope_args
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "rapport_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(?:int|const char\*)\s+(fp_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from rapport_amd import _lib
+    lib = _lib.load()
+    decl = _declared()
+    assert decl == _lib.declared_symbols()
+    for name in decl:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    exported = set(re.findall(r" T (fp_\w+)", out))
+    assert set(decl) <= exported
+    assert lib.fp_version() >= 100
+
+
+def test_gpu_entry_points_refuse_cpu_tensors():
+    import torch
+    it_cfg = R.Config(**cfg_kwargs(C1))
+    g = R.build_grids(it_cfg)
+    from rapport_amd import _lib
+    with pytest.raises(ValueError):
+        _lib.ptr(torch.zeros(3))
+
+
+@pytest.mark.parametrize("case", [C1, C2])
+def test_config_and_grids_match_oracle(case):
+    kw = cfg_kwargs(case)
+    cp, co = R.Config(**kw), O.Config(**kw)
+    for a in ("omega0", "n0", "chi3", "komega", "dz", "b", "E0", "I0_Wcm2", "Omega_r2", "nc_probe"):
+        assert np.isclose(getattr(cp, a), getattr(co, a), rtol=1e-15, atol=0), a
+    gp, go = R.build_grids(cp), O.build_grids(co)
+    for k in ("tmax", "dt", "tlist", "ff", "Y", "R", "j1last", "rlist", "rholist", "delta_k", "k1",
+              "T_op", "inv_U_op", "inv_U_nl", "R_f", "spec_mask", "rr", "tt", "rhorho",
+              "sigmaomega", "avalanche_coef", "avalanche_coef_s", "inv_taur_eff", "inv_tau_ste",
+              "ste_pref", "f_ste", "invE2", "mask_r", "b", "E0", "komega"):
+        a, b = np.asarray(gp[k]), np.asarray(go[k])
+        assert a.shape == b.shape, k
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= 1e-13 * scale, k
+    kp, ko = gp["keldysh"], go["keldysh"]
+    for k in ("W_LUT", "W_LUT_s", "NLUT", "logImin", "inv_dlog", "Imin", "Imax"):
+        assert np.allclose(kp[k], ko[k], rtol=1e-14, atol=0), k
+    assert np.array_equal(kp["f_spline"].x, ko["f_spline"].x)
+    assert np.allclose(kp["f_spline"].c, ko["f_spline"].c, rtol=1e-14, atol=0)
+
+
+def test_envelopes_match_oracle():
+    kw = cfg_kwargs(C2)
+    cp, co = R.Config(**kw), O.Config(**kw)
+    gp, go = R.build_grids(cp), O.build_grids(co)
+    up = R.envelope_lens_noise(**envelope_args(C2))(gp["rr"], gp["tt"], cp, gp)
+    uo = O.envelope_lens_noise(**envelope_args(C2))(go["rr"], go["tt"], co, go)
+    assert np.abs(up - uo).max() <= 1e-14 * np.abs(uo).max()
+    u1 = R.envelope_gaussian_focused(gp["rr"], gp["tt"], cp, gp)
+    r0 = gp["rlist"][0]
+    assert np.abs(u1[0]).max() == pytest.approx(cp.E0 * np.exp(-r0 ** 2 / cp.w0 ** 2), rel=1e-12)
+
+
+def test_keldysh_validate_product():
+    from rapport_amd import keldysh
+    v = keldysh.validate(verbose=False)
+    assert f"{v['W_3.5e13_beta4']:.2e}" == "6.44e+32"
+    assert f"{v['W_3.5e13_beta2']:.2e}" == "4.30e+32"
+    assert np.allclose(keldysh.keldysh_multiphoton(np.array([1e11])),
+                       O.Keldysh(800e-9, 9.0, 0.64 * 9.1093837015e-31, O.n_sellmeier(800e-9)).rate(np.array([1e11])),
+                       rtol=0.3)
+
+
+def test_fingerprint_changes_with_sources(tmp_path):
+    fp = R.code_fingerprint()
+    assert re.fullmatch(r"[0-9a-f]{12}", fp)
