@@ -107,6 +107,18 @@ def run_cpu_baseline(n_steps=2):
                        f"FP64 oracle, BLAS/FFT threads = {_cores()}")
 
 
+def copy_calibration(nbytes=256 * 2 ** 20, reps=5):
+    """A plain device-to-device copy of a known size: the FETCH_SIZE / WRITE_SIZE
+    counters of its kernel calibrate the counters read for the propagator kernels."""
+    import torch
+    x = torch.rand(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(copy_bytes=nbytes, reps=reps)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,7 +126,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the propagator is a single-GPU build); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
