@@ -68,14 +68,16 @@ typedef struct fp_nl_tables {
 } fp_nl_tables;
 
 /* Same tables in float32 for the mixed-precision path. PCHIP coefficients are
- * in the local coordinate s = (I - x_i)/h_i and pre-scaled by photo_pref. */
+ * in the local coordinate s = (I - x_i)/h_i and pre-scaled by photo_pref, so
+ * photo = |P(s)| * depl / (|u|^2 + 1e-30) stays inside float range. */
 typedef struct fp_nl_tables_f32 {
-  const float* tw;
+  const float* tw;        /* Nt complex64 twiddles */
   const float* mA;
   const float* mB;
   const float* mC;
-  const float* Rf;
-  const double* knots;
+  const float* Rf;        /* Nt complex64 */
+  const float* knots;     /* n_knots */
+  const float* inv_h;     /* n_knots-1: 1 / (x_{i+1} - x_i) */
   const float* pcoef_s;   /* 4 x (n_knots-1) */
 } fp_nl_tables_f32;
 
@@ -122,9 +124,10 @@ int fp_hankel_gemm(const double* Yt, const double* B, double* C, int Nr, int Nt,
 
 /* ---------------------------------------------------------------------------
  * LinearOperator.half_linear (D3/sim/operators.py:47-53) on time-domain rows:
- * out = iRk * Y @ ifft(phase_half * fft(Rk * Y @ u)), with
- * phase_half[m, f] = Rk*iRk... folded: phase_half = exp(i phi dz/2) * spec_mask.
- * work: 2 x Nr x Nt complex128 scratch. u may equal out. */
+ * out = ifft_t(iRk * Y @ (phase_half o (Rk * Y @ fft_t(u)))), which equals the
+ * reference's iRk Y ifft(phase fft(Rk Y u)) because Y (along r) commutes with
+ * the FFT (along t). phase_half[m, f] = exp(i phi[m, f] dz/2) * spec_mask[f]
+ * (Nr x Nt complex128). work: 2 x Nr x Nt complex128 scratch. u may equal out. */
 int fp_half_linear(const double* u, double* out, int Nr, int Nt, const double* Yt,
                    const double* phase_half, double Rk, double iRk, const double* tw,
                    double* work, void* stream);
@@ -158,6 +161,50 @@ int fp_nl_step(const double* in, double* out, const double* rho, const double* r
 int fp_record_rows(const double* u, const double* rho, const double* rho_s, double* rows_out,
                    const fp_nl_params* p, const fp_nl_tables* t, double dt, int nbins,
                    const int* bin_lo, const int* bin_hi, double* bins, void* stream);
+
+/* ===========================================================================
+ * Mixed-precision path (Config.precision = "mixed"): complex64 fields, FP32
+ * transforms and RK4, split-FP16 Hankel GEMM with FP32 accumulation, FP64
+ * carrier densities. Same operations and layouts as the FP64 entry points.
+ * "slots" are device words holding max|x| of a buffer as float bits (written
+ * with atomicMax, cleared by the kernels named below), used to pick the
+ * power-of-two FP16 scale of the next GEMM without a host round trip.
+ * ======================================================================== */
+
+/* max(|re|, |im|) over n floats into *slot (atomicMax of float bits). */
+int fp_absmax_f32(const float* x, long long n, unsigned* slot, void* stream);
+
+/* Batched complex64 FFT along t (cf. fp_fft_rows). */
+int fp_fft_rows_c64(const float* in, float* out, int Nr, int Nt, int inverse, const float* tw,
+                    void* stream);
+
+/* Carrier rate equations (FP64) from complex64 spectral rows (cf.
+ * fp_plasma_scan_spec; replaces update_plasma, D3/sim/operators.py:165-183).
+ * clear_slots (optional): 3 consecutive slot words set to 0 by the kernel. */
+int fp_plasma_scan_spec_c64(const float* spec, double* rho, double* rho_s, float* u_out,
+                            const fp_plasma_params* p, const double* W_lut, const double* Ws_lut,
+                            const float* tw, unsigned* clear_slots, int* status, void* stream);
+
+/* Hankel GEMM, split FP16 x 3 on MFMA (replaces cp.dot(self.Y, u),
+ * D3/sim/operators.py:48,53). Yh/Yl: Y * 2^y_scale_exp split into FP16 hi/lo in
+ * fragment order (see rapport_amd/mixed.py). B, C: Nr x Nt complex64. bmax_in:
+ * slot holding max|B|; cmax_out (optional): slot receiving max|C|; reset_slot
+ * (optional): cleared by the kernel before any of its blocks can write it.
+ * Requires Nr % 128 == 0 and (2 Nt) % 128 == 0. */
+int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const float* B,
+                         float* C, int Nr, int Nt, float alpha, const float* rowscale,
+                         const float* phase, const unsigned* bmax_in, unsigned* cmax_out,
+                         unsigned* reset_slot, void* stream);
+
+/* split() on complex64 rows (cf. fp_nl_split). */
+int fp_nl_split_c64(const float* u, const double* rho, const double* rho_s, float* rhs,
+                    float* alpha, const fp_nl_params* p, const fp_nl_tables_f32* t,
+                    void* stream);
+
+/* Fused nonlinear step on complex64 rows (cf. fp_nl_step); cmax_out optional. */
+int fp_nl_step_c64(const float* in, float* out, const double* rho, const double* rho_s,
+                   int in_spectral, int out_spectral, const fp_nl_params* p,
+                   const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream);
 
 #ifdef __cplusplus
 }

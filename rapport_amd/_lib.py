@@ -42,7 +42,7 @@ class NLTables(ctypes.Structure):
 class NLTablesF32(ctypes.Structure):
     """fp_nl_tables_f32."""
     _fields_ = [("tw", P), ("mA", P), ("mB", P), ("mC", P), ("Rf", P), ("knots", P),
-                ("pcoef_s", P)]
+                ("inv_h", P), ("pcoef_s", P)]
 
 
 # name -> argtypes (restype is int for all but fp_last_error)
@@ -57,6 +57,13 @@ _SIGS = {
     "fp_nl_step": [P, P, P, P, c_int, c_int, ctypes.POINTER(NLParams), ctypes.POINTER(NLTables), P],
     "fp_record_rows": [P, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTables), c_double,
                        c_int, P, P, P, P],
+    "fp_absmax_f32": [P, ctypes.c_longlong, P, P],
+    "fp_fft_rows_c64": [P, P, c_int, c_int, c_int, P, P],
+    "fp_plasma_scan_spec_c64": [P, P, P, P, ctypes.POINTER(PlasmaParams), P, P, P, P, P, P],
+    "fp_hankel_gemm_mixed": [P, P, c_int, P, P, c_int, c_int, ctypes.c_float, P, P, P, P, P, P],
+    "fp_nl_split_c64": [P, P, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTablesF32), P],
+    "fp_nl_step_c64": [P, P, P, P, c_int, c_int, ctypes.POINTER(NLParams),
+                       ctypes.POINTER(NLTablesF32), P, P],
 }
 
 _lib = None

@@ -90,6 +90,7 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
     for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
     if (Ns > 1) {
       if (sizeof(T) == 8) {
+        // FP64: powers of one table twiddle (rounding growth ~R ulp is below FP64 needs)
         cpx<T> w1 = tw[k * STRIDE];
         if (INV) w1 = conjx(w1);
         cpx<T> w = w1;
@@ -99,12 +100,28 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
           if (r + 1 < R) w = w * w1;
         }
       } else {
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          cpx<T> w = tw[k * r * STRIDE];
-          if (INV) w = conjx(w);
-          a[r] = a[r] * w;
+        // FP32: table loads of w^1, w^4 (and w^8 for R = 16); the rest by at most
+        // two products, keeping every twiddle within a few ulp.
+        cpx<T> w[R];
+        w[0] = mk<T>(T(1), T(0));
+        w[1] = tw[k * STRIDE];
+        if (R >= 4) {
+          w[2] = w[1] * w[1];
+          w[3] = w[2] * w[1];
         }
+        if (R >= 8) {
+          w[4] = tw[4 * k * STRIDE];
+          w[5] = w[4] * w[1];
+          w[6] = w[4] * w[2];
+          w[7] = w[4] * w[3];
+        }
+        if (R >= 16) {
+          w[8] = tw[8 * k * STRIDE];
+#pragma unroll
+          for (int r = 9; r < 16; ++r) w[r] = w[8] * w[r - 8];
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) a[r] = a[r] * (INV ? conjx(w[r]) : w[r]);
       }
     }
     dft<T, R, INV>(a);
@@ -131,7 +148,12 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
 template <typename T, int N, int TPB, bool INV>
 FP_DEV void row_fft(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw) {
   static_assert(N % TPB == 0, "N must be a multiple of TPB");
-  passes<T, N, TPB, INV, 1>(v, lds, tw, (int)threadIdx.x);
+  // Opaque copy of the thread index: keeps the compiler from hoisting this
+  // transform's LDS / twiddle addresses out of enclosing loops and holding
+  // them in registers across the whole kernel.
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  passes<T, N, TPB, INV, 1>(v, lds, tw, tid);
 }
 
 }  // namespace fpfft

@@ -22,14 +22,14 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 128, BN = 128, BK = 16, NTHR = 256;
 constexpr int APAD = 4;  // LDS row padding (doubles)
 
-__global__ __launch_bounds__(NTHR) void hankel_gemm_f64(const double* __restrict__ Yt,
+__global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64(const double* __restrict__ Yt,
                                                          const double* __restrict__ B,
                                                          double* __restrict__ C, int M, int K,
                                                          int Ncols, double alpha,
                                                          const double* __restrict__ rowscale,
                                                          const double* __restrict__ phase) {
-  __shared__ double As[2][BK][BM + APAD];
-  __shared__ double Bs[2][BK][BN + APAD];
+  __shared__ __attribute__((aligned(16))) double As[2][BK][BM + APAD];
+  __shared__ __attribute__((aligned(16))) double Bs[2][BK][BN + APAD];
 
   const int tilesM = (M + BM - 1) / BM;
   const int tilesN = (Ncols + BN - 1) / BN;
@@ -45,27 +45,36 @@ __global__ __launch_bounds__(NTHR) void hankel_gemm_f64(const double* __restrict
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
 
-  // global->LDS staging: each thread moves 8 doubles of A and 8 of B per slab.
-  // A slab: rows k0..k0+15 of Yt, cols m0..m0+127 -> thread: k = tid/16, 8 cols at (tid%16)*8
+  // global->LDS staging: each thread moves 8 consecutive doubles of A (row ldk
+  // of the Y^T slab) and 8 of B per slab; full tiles use 16-byte loads/stores.
   const int ldk = tid >> 4;
   const int ldc = (tid & 15) * 8;
-  double ra[8], rb[8];
+  const bool full = (m0 + BM <= M) && (n0 + BN <= Ncols) && (K % BK == 0);
+  double2 ra[4], rb[4];
 
   auto gload = [&](int k0) {
     const int k = k0 + ldk;
+    if (full) {
+      const double2* pa = reinterpret_cast<const double2*>(Yt + (size_t)k * M + m0 + ldc);
+      const double2* pb = reinterpret_cast<const double2*>(B + (size_t)k * Ncols + n0 + ldc);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int m = m0 + ldc + i, n = n0 + ldc + i;
-      ra[i] = (k < K && m < M) ? Yt[(size_t)k * M + m] : 0.0;
-      rb[i] = (k < K && n < Ncols) ? B[(size_t)k * Ncols + n] : 0.0;
+      for (int i = 0; i < 4; ++i) { ra[i] = pa[i]; rb[i] = pb[i]; }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = m0 + ldc + 2 * i, n = n0 + ldc + 2 * i;
+        ra[i].x = (k < K && m < M) ? Yt[(size_t)k * M + m] : 0.0;
+        ra[i].y = (k < K && m + 1 < M) ? Yt[(size_t)k * M + m + 1] : 0.0;
+        rb[i].x = (k < K && n < Ncols) ? B[(size_t)k * Ncols + n] : 0.0;
+        rb[i].y = (k < K && n + 1 < Ncols) ? B[(size_t)k * Ncols + n + 1] : 0.0;
+      }
     }
   };
   auto sstore = [&](int buf) {
+    double2* sa = reinterpret_cast<double2*>(&As[buf][ldk][ldc]);
+    double2* sb = reinterpret_cast<double2*>(&Bs[buf][ldk][ldc]);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      As[buf][ldk][ldc + i] = ra[i];
-      Bs[buf][ldk][ldc + i] = rb[i];
-    }
+    for (int i = 0; i < 4; ++i) { sa[i] = ra[i]; sb[i] = rb[i]; }
   };
 
   d4 acc[4][4];

@@ -18,21 +18,24 @@ using fpfft::LdsSize;
 using fpfft::row_fft;
 
 // Exact decade boundaries of the PCHIP knot grid 10^k (1 + 0.01 j).
-FP_DEV double p10(int k) {
-  constexpr double t[18] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8,
-                            1e9, 1e10, 1e11, 1e12, 1e13, 1e14, 1e15, 1e16, 1e17};
-  return t[k];
-}
+__device__ const double kPow10[18] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8,
+                                      1e9, 1e10, 1e11, 1e12, 1e13, 1e14, 1e15, 1e16, 1e17};
+__device__ const double kInvPow10[18] = {1e0, 1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8,
+                                         1e-9, 1e-10, 1e-11, 1e-12, 1e-13, 1e-14, 1e-15, 1e-16,
+                                         1e-17};
+FP_DEV double p10(int k) { return kPow10[k]; }
 
 // Interval i with knots[i] <= x < knots[i+1] (x in [knots[0], knots[n-1]];
 // x == knots[n-1] maps to n-2, as scipy.interpolate.PPoly does).
 FP_DEV int pchip_interval(double x, const double* __restrict__ knots, int n) {
-  int k = (int)log10(x);
-  if (k < 0) k = 0;
+  // decade from the binary exponent: floor(e2 log10 2) = (e2 * 1233) >> 12 for
+  // 0 <= e2 < 1024, then an exact fix-up against the powers of ten
+  const int e2 = (int)((__double_as_longlong(x) >> 52) & 0x7ff) - 1023;
+  int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;
   if (k > 16) k = 16;
   while (k < 16 && x >= p10(k + 1)) ++k;
   while (k > 0 && x < p10(k)) --k;
-  int j = (int)((x / p10(k) - 1.0) * 100.0);
+  int j = (int)((x * kInvPow10[k] - 1.0) * 100.0);
   if (j < 0) j = 0;
   if (j > 899) j = 899;
   int i = k * 900 + j;
@@ -73,63 +76,61 @@ FP_DEV void photo_alpha(double absu2, double rho, const fp_nl_params& p, const f
   alpha = a > 0.0 ? a : 0.0;
 }
 
-// One split() evaluation on a row in registers: k = rhs(us), al = alpha(us).
+// One split() evaluation on a row in registers: k = rhs(us) and, if al is
+// given, al = alpha(us). Register state is kept to us and two transform
+// buffers: rho (and rho_s) are re-read from the row in global memory (L2
+// resident), |us|^2 is recomputed where needed, the Kerr intensity is formed in
+// the transform buffer, and alpha is re-evaluated for the final alpha*u term.
 template <int NT, int TPB, bool RS>
-FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double (&rho)[NT / TPB],
-                      const double (&rhos)[RS ? NT / TPB : 1], cpx<double> (&k)[NT / TPB],
-                      double (&al)[NT / TPB], const fp_nl_params& p, const fp_nl_tables& t,
+FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double* __restrict__ rho_row,
+                      const double* __restrict__ rhos_row, cpx<double> (&k)[NT / TPB],
+                      double* al, const fp_nl_params& p, const fp_nl_tables& t,
                       cpx<double>* lds) {
   constexpr int E = NT / TPB;
-  const int tid = threadIdx.x;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));   // recompute table addresses here rather than hoist them
   const cpx<double>* tw = reinterpret_cast<const cpx<double>*>(t.tw);
   const cpx<double>* Rf = reinterpret_cast<const cpx<double>*>(t.Rf);
   const double invN = 1.0 / (double)NT;
-  double absu2[E];
-  unsigned zmask = 0;
-  cpx<double> buf[E], nl[E];
-  // pointwise terms; B = photo * u goes straight into the transform buffer
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
-    absu2[s] = a2;
-    if (!(a2 >= 1e-30)) zmask |= 1u << s;   // where(absu2 < 1e-30) (NaN -> nan_to_num 0 anyway)
-    double ph, a;
-    photo_alpha(a2, rho[s], p, t, ph, a);
-    al[s] = a;
-    buf[s] = ph * us[s];
-  }
-  const bool any_photo = p.en_pi_loss;
-  if (any_photo) {
-    row_fft<double, NT, TPB, false>(buf, lds, tw);
-#pragma unroll
-    for (int s = 0; s < E; ++s) nl[s] = t.mB[tid + s * TPB] * buf[s];
-  } else {
-#pragma unroll
-    for (int s = 0; s < E; ++s) nl[s] = mk<double>(0.0, 0.0);
-  }
-  // Kerr intensity with the Raman convolution
-  double kerr[E];
-#pragma unroll
-  for (int s = 0; s < E; ++s) kerr[s] = p.en_kerr_inst ? (1.0 - p.f_R) * absu2[s] : 0.0;
-  if (p.en_kerr_raman) {
-#pragma unroll
-    for (int s = 0; s < E; ++s) buf[s] = mk<double>(absu2[s], 0.0);
-    row_fft<double, NT, TPB, false>(buf, lds, tw);
-#pragma unroll
-    for (int s = 0; s < E; ++s) buf[s] = buf[s] * Rf[tid + s * TPB];
-    row_fft<double, NT, TPB, true>(buf, lds, tw);
-#pragma unroll
-    for (int s = 0; s < E; ++s) kerr[s] = kerr[s] + p.f_R * (buf[s].x * invN);
-  }
-  if (p.en_kerr_inst || p.en_kerr_raman) {
+  cpx<double> buf[E];
+  // k doubles as the spectral accumulator of the three weighted transforms.
+  // B = photo * u
+  if (p.en_pi_loss) {
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const double kk = p.kerr_pref * kerr[s];
-      buf[s] = mk<double>(-kk * us[s].y, kk * us[s].x);   // i * kappa * kerr_I * u
+      const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
+      double ph, a;
+      photo_alpha(a2, rho_row[tid + s * TPB], p, t, ph, a);
+      buf[s] = ph * us[s];
     }
     row_fft<double, NT, TPB, false>(buf, lds, tw);
 #pragma unroll
-    for (int s = 0; s < E; ++s) nl[s] = nl[s] + t.mA[tid + s * TPB] * buf[s];
+    for (int s = 0; s < E; ++s) k[s] = t.mB[tid + s * TPB] * buf[s];
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) k[s] = mk<double>(0.0, 0.0);
+  }
+  // Kerr intensity (instantaneous + Raman convolution), then A = i kappa kerr_I u
+  if (p.en_kerr_inst || p.en_kerr_raman) {
+    if (p.en_kerr_raman) {
+#pragma unroll
+      for (int s = 0; s < E; ++s) buf[s] = mk<double>(us[s].x * us[s].x + us[s].y * us[s].y, 0.0);
+      row_fft<double, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+      for (int s = 0; s < E; ++s) buf[s] = buf[s] * Rf[tid + s * TPB];
+      row_fft<double, NT, TPB, true>(buf, lds, tw);
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
+      double kerr = p.en_kerr_inst ? (1.0 - p.f_R) * a2 : 0.0;
+      if (p.en_kerr_raman) kerr = kerr + p.f_R * (buf[s].x * invN);
+      const double kk = p.kerr_pref * kerr;
+      buf[s] = mk<double>(-kk * us[s].y, kk * us[s].x);
+    }
+    row_fft<double, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) k[s] = k[s] + t.mA[tid + s * TPB] * buf[s];
   }
   // extra = -pp * pc * rho * u + i * ste * rho_s * u
   const bool any_extra = (p.pc_re != 0.0 || p.pc_im != 0.0) || (RS && p.ste_pref != 0.0);
@@ -137,22 +138,26 @@ FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double (&rho)[NT 
     const cpx<double> g = mk<double>(-p.plasma_pref * p.pc_re, -p.plasma_pref * p.pc_im);
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      cpx<double> e = rho[s] * (g * us[s]);
+      cpx<double> e = rho_row[tid + s * TPB] * (g * us[s]);
       if (RS && p.ste_pref != 0.0) {
-        const double q = p.ste_pref * rhos[s];
+        const double q = p.ste_pref * rhos_row[tid + s * TPB];
         e = e + mk<double>(-q * us[s].y, q * us[s].x);
       }
       buf[s] = e;
     }
     row_fft<double, NT, TPB, false>(buf, lds, tw);
 #pragma unroll
-    for (int s = 0; s < E; ++s) nl[s] = nl[s] + t.mC[tid + s * TPB] * buf[s];
+    for (int s = 0; s < E; ++s) k[s] = k[s] + t.mC[tid + s * TPB] * buf[s];
   }
-  row_fft<double, NT, TPB, true>(nl, lds, tw);
+  row_fft<double, NT, TPB, true>(k, lds, tw);
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    cpx<double> r = mk<double>(nl[s].x * invN + al[s] * us[s].x, nl[s].y * invN + al[s] * us[s].y);
-    if (zmask & (1u << s)) r = mk<double>(0.0, 0.0);
+    const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
+    double ph, a;
+    photo_alpha(a2, rho_row[tid + s * TPB], p, t, ph, a);
+    if (al) al[s] = a;
+    cpx<double> r = mk<double>(k[s].x * invN + a * us[s].x, k[s].y * invN + a * us[s].y);
+    if (!(a2 >= 1e-30)) r = mk<double>(0.0, 0.0);   // where(absu2 < 1e-30, 0, rhs)
     k[s] = mk<double>(nan0(r.x), nan0(r.y));
   }
 }
@@ -182,15 +187,9 @@ __global__ __launch_bounds__(TPB) void split_kernel(const double* __restrict__ u
   constexpr int E = NT / TPB;
   const size_t row = blockIdx.x;
   cpx<double> us[E], k[E];
-  double r[E], rs[RS ? E : 1], al[E];
+  double al[E];
   load_row<NT, TPB>(u, row, us);
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    const size_t i = row * NT + threadIdx.x + s * TPB;
-    r[s] = rho[i];
-    if (RS) rs[s] = rhos[i];
-  }
-  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
+  split_row<NT, TPB, RS>(us, rho + row * NT, RS ? rhos + row * NT : nullptr, k, al, p, t, lds);
   store_row<NT, TPB>(rhs, row, k);
   if (alpha) {
 #pragma unroll
@@ -199,14 +198,14 @@ __global__ __launch_bounds__(TPB) void split_kernel(const double* __restrict__ u
 }
 
 // alpha half-step: u *= exp(-0.5 dz alpha(u, rho))
-template <int E>
-FP_DEV void alpha_half(cpx<double> (&u)[E], const double (&rho)[E], const fp_nl_params& p,
-                       const fp_nl_tables& t) {
+template <int E, int TPB>
+FP_DEV void alpha_half(cpx<double> (&u)[E], const double* __restrict__ rho_row,
+                       const fp_nl_params& p, const fp_nl_tables& t) {
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const double a2 = u[s].x * u[s].x + u[s].y * u[s].y;
     double ph, a;
-    photo_alpha(a2, rho[s], p, t, ph, a);
+    photo_alpha(a2, rho_row[threadIdx.x + s * TPB], p, t, ph, a);
     const double f = exp(-0.5 * p.dz * a);
     u[s] = f * u[s];
   }
@@ -234,48 +233,28 @@ __global__ __launch_bounds__(TPB) void nl_step_kernel(const double* in, double* 
 #pragma unroll
     for (int s = 0; s < E; ++s) us[s] = invN * us[s];
   }
-  double r[E], rs[RS ? E : 1];
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    const size_t i = row * NT + tid + s * TPB;
-    r[s] = rho[i];
-    if (RS) rs[s] = rhos[i];
-  }
-  alpha_half<E>(us, r, p, t);
+  const double* rr = rho + row * NT;
+  const double* rs = RS ? rhos + row * NT : nullptr;
+  alpha_half<E, TPB>(us, rr, p, t);
 #pragma unroll
   for (int s = 0; s < E; ++s) ub[tid + s * TPB] = us[s];
   const double dz = p.dz;
   cpx<double> acc[E], k[E];
-  double al[E];
-  // k1 (us == u)
-  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
+  // classical RK4 with rho frozen: k1 = f(u), k2 = f(u + dz/2 k1), k3 = f(u + dz/2 k2),
+  // k4 = f(u + dz k3), u += dz/6 (k1 + 2 k2 + 2 k3 + k4). One split_row body in the
+  // instruction stream: the stage loop is not unrolled.
+#pragma unroll 1
+  for (int stg = 0; stg < 4; ++stg) {
+    split_row<NT, TPB, RS>(us, rr, rs, k, nullptr, p, t, lds);
+    const double w = (stg == 0 || stg == 3) ? 1.0 : 2.0;
+    const double cdz = (stg < 2) ? 0.5 * dz : dz;
 #pragma unroll
-  for (int s = 0; s < E; ++s) {
-    acc[s] = k[s];
-    us[s] = ub[tid + s * TPB] + (0.5 * dz) * k[s];
+    for (int s = 0; s < E; ++s) {
+      acc[s] = (stg == 0) ? k[s] : acc[s] + w * k[s];
+      us[s] = (stg < 3) ? ub[tid + s * TPB] + cdz * k[s] : ub[tid + s * TPB] + (dz / 6.0) * acc[s];
+    }
   }
-  // k2
-  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    acc[s] = acc[s] + 2.0 * k[s];
-    us[s] = ub[tid + s * TPB] + (0.5 * dz) * k[s];
-  }
-  // k3
-  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    acc[s] = acc[s] + 2.0 * k[s];
-    us[s] = ub[tid + s * TPB] + dz * k[s];
-  }
-  // k4
-  split_row<NT, TPB, RS>(us, r, rs, k, al, p, t, lds);
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    acc[s] = acc[s] + k[s];
-    us[s] = ub[tid + s * TPB] + (dz / 6.0) * acc[s];
-  }
-  alpha_half<E>(us, r, p, t);
+  alpha_half<E, TPB>(us, rr, p, t);
   if (out_spectral) row_fft<double, NT, TPB, false>(us, lds, tw);
   store_row<NT, TPB>(out, row, us);
 }

@@ -1,0 +1,239 @@
+// Nonlinear operator of the split step in FP32 (mixed-precision path), one
+// workgroup per radial row: the same arithmetic as nl.hpp (split() of
+// D3/sim/operators.py:89-146 and the nonlinear part of Integrator.step,
+// D3/sim/integrator.py:141-147) with complex64 rows, FP32 transforms and an
+// FP32 PCHIP. The carrier densities stay FP64 in memory and are rounded when
+// read. The field-loss PCHIP is evaluated in the local coordinate
+// s = (I - x_i)/h_i with coefficients pre-multiplied by photo_pref, so every
+// intermediate stays inside FP32 range (|W photo_pref| < 6e30 over the table).
+#pragma once
+#include "common.hpp"
+#include "fft.hpp"
+#include "../../include/rapport_amd.h"
+
+namespace fpnl32 {
+
+using fpfft::LdsSize;
+using fpfft::row_fft;
+typedef cpx<float> c32;
+
+// |W(I)| * photo_pref * depl / (|u|^2 + 1e-30), the photo-ionisation loss rate.
+FP_DEV float photo_rate(float a2, float rho, const fp_nl_params& p, const fp_nl_tables_f32& t) {
+  float x = a2 * (float)p.invE2;
+  x = fminf(fmaxf(x, (float)p.Imin), (float)p.Imax);
+  const float* kn = t.knots;
+  const int n = p.n_knots;
+  const int e2 = (int)((__float_as_uint(x) >> 23) & 0xff) - 127;
+  int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;
+  if (k > 16) k = 16;
+  while (k < 16 && x >= kn[(k + 1) * 900]) ++k;
+  while (k > 0 && x < kn[k * 900]) --k;
+  int j = (int)((x / kn[k * 900] - 1.0f) * 100.0f);
+  if (j < 0) j = 0;
+  if (j > 899) j = 899;
+  int i = k * 900 + j;
+  if (i > n - 2) i = n - 2;
+  while (i < n - 2 && kn[i + 1] <= x) ++i;
+  while (i > 0 && kn[i] > x) --i;
+  const int m = n - 1;
+  const float s = (x - kn[i]) * t.inv_h[i];
+  const float w = fabsf(((t.pcoef_s[i] * s + t.pcoef_s[m + i]) * s + t.pcoef_s[2 * m + i]) * s +
+                        t.pcoef_s[3 * m + i]);
+  float depl = 1.0f - rho / (float)p.rho_max;
+  depl = fminf(fmaxf(depl, 0.0f), 1.0f);
+  return nan0(w) * depl / (a2 + 1e-30f);
+}
+
+FP_DEV float alpha_of(float a2, float rho, const fp_nl_params& p, const fp_nl_tables_f32& t,
+                      float& photo) {
+  photo = p.en_pi_loss ? photo_rate(a2, rho, p, t) : 0.0f;
+  float a = photo;
+  if (p.en_plasma_absorb) a = a + (float)p.plasma_pref * rho;
+  a = nan0(a);
+  return a > 0.0f ? a : 0.0f;
+}
+
+template <int NT, int TPB, bool RS>
+FP_DEV void split_row(const c32 (&us)[NT / TPB], const double* __restrict__ rho_row,
+                      const double* __restrict__ rhos_row, c32 (&k)[NT / TPB], float* al,
+                      const fp_nl_params& p, const fp_nl_tables_f32& t, c32* lds) {
+  constexpr int E = NT / TPB;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const c32* tw = reinterpret_cast<const c32*>(t.tw);
+  const c32* Rf = reinterpret_cast<const c32*>(t.Rf);
+  const float invN = 1.0f / (float)NT;
+  c32 buf[E];
+  if (p.en_pi_loss) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
+      const float ph = photo_rate(a2, (float)rho_row[tid + s * TPB], p, t);
+      buf[s] = ph * us[s];
+    }
+    row_fft<float, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) k[s] = t.mB[tid + s * TPB] * buf[s];
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) k[s] = mk<float>(0.f, 0.f);
+  }
+  if (p.en_kerr_inst || p.en_kerr_raman) {
+    if (p.en_kerr_raman) {
+#pragma unroll
+      for (int s = 0; s < E; ++s) buf[s] = mk<float>(us[s].x * us[s].x + us[s].y * us[s].y, 0.f);
+      row_fft<float, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+      for (int s = 0; s < E; ++s) buf[s] = buf[s] * Rf[tid + s * TPB];
+      row_fft<float, NT, TPB, true>(buf, lds, tw);
+    }
+    const float fR = (float)p.f_R, kp = (float)p.kerr_pref;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
+      float kerr = p.en_kerr_inst ? (1.0f - fR) * a2 : 0.0f;
+      if (p.en_kerr_raman) kerr = kerr + fR * (buf[s].x * invN);
+      const float kk = kp * kerr;
+      buf[s] = mk<float>(-kk * us[s].y, kk * us[s].x);
+    }
+    row_fft<float, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) k[s] = k[s] + t.mA[tid + s * TPB] * buf[s];
+  }
+  const bool any_extra = (p.pc_re != 0.0 || p.pc_im != 0.0) || (RS && p.ste_pref != 0.0);
+  if (any_extra) {
+    const c32 g = mk<float>((float)(-p.plasma_pref * p.pc_re), (float)(-p.plasma_pref * p.pc_im));
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      c32 e = (float)rho_row[tid + s * TPB] * (g * us[s]);
+      if (RS && p.ste_pref != 0.0) {
+        const float q = (float)(p.ste_pref * rhos_row[tid + s * TPB]);
+        e = e + mk<float>(-q * us[s].y, q * us[s].x);
+      }
+      buf[s] = e;
+    }
+    row_fft<float, NT, TPB, false>(buf, lds, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) k[s] = k[s] + t.mC[tid + s * TPB] * buf[s];
+  }
+  row_fft<float, NT, TPB, true>(k, lds, tw);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
+    float ph;
+    const float a = alpha_of(a2, (float)rho_row[tid + s * TPB], p, t, ph);
+    if (al) al[s] = a;
+    c32 r = mk<float>(k[s].x * invN + a * us[s].x, k[s].y * invN + a * us[s].y);
+    if (!(a2 >= 1e-30f)) r = mk<float>(0.f, 0.f);
+    k[s] = mk<float>(nan0(r.x), nan0(r.y));
+  }
+}
+
+template <int NT, int TPB>
+FP_DEV void load_row(const float* __restrict__ src, size_t row, c32 (&v)[NT / TPB]) {
+  const c32* r = reinterpret_cast<const c32*>(src) + row * NT;
+#pragma unroll
+  for (int s = 0; s < NT / TPB; ++s) v[s] = r[threadIdx.x + s * TPB];
+}
+template <int NT, int TPB>
+FP_DEV void store_row(float* __restrict__ dst, size_t row, const c32 (&v)[NT / TPB]) {
+  c32* r = reinterpret_cast<c32*>(dst) + row * NT;
+#pragma unroll
+  for (int s = 0; s < NT / TPB; ++s) r[threadIdx.x + s * TPB] = v[s];
+}
+
+template <int NT, int TPB, bool RS>
+__global__ __launch_bounds__(TPB) void split_kernel(const float* __restrict__ u,
+                                                     const double* __restrict__ rho,
+                                                     const double* __restrict__ rhos,
+                                                     float* __restrict__ rhs,
+                                                     float* __restrict__ alpha, fp_nl_params p,
+                                                     fp_nl_tables_f32 t) {
+  __shared__ c32 lds[LdsSize<NT>::value];
+  constexpr int E = NT / TPB;
+  const size_t row = blockIdx.x;
+  c32 us[E], k[E];
+  float al[E];
+  load_row<NT, TPB>(u, row, us);
+  split_row<NT, TPB, RS>(us, rho + row * NT, RS ? rhos + row * NT : nullptr, k, al, p, t, lds);
+  store_row<NT, TPB>(rhs, row, k);
+  if (alpha) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) alpha[row * NT + threadIdx.x + s * TPB] = al[s];
+  }
+}
+
+template <int E, int TPB>
+FP_DEV void alpha_half(c32 (&u)[E], const double* __restrict__ rho_row, const fp_nl_params& p,
+                       const fp_nl_tables_f32& t) {
+  const float hdz = (float)(-0.5 * p.dz);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const float a2 = u[s].x * u[s].x + u[s].y * u[s].y;
+    float ph;
+    const float a = alpha_of(a2, (float)rho_row[threadIdx.x + s * TPB], p, t, ph);
+    u[s] = expf(hdz * a) * u[s];
+  }
+}
+
+// Fused nonlinear step (alpha/2, RK4, alpha/2) on complex64 rows; spectral in
+// and out when the flags are set. cmax_out (optional) receives max |re|, |im|
+// of the output rows (float bits, atomicMax) for the next split-FP16 GEMM.
+template <int NT, int TPB, bool RS>
+__global__ __launch_bounds__(TPB) void nl_step_kernel(const float* in, float* out,
+                                                       const double* __restrict__ rho,
+                                                       const double* __restrict__ rhos,
+                                                       int in_spectral, int out_spectral,
+                                                       fp_nl_params p, fp_nl_tables_f32 t,
+                                                       unsigned* __restrict__ cmax_out) {
+  __shared__ c32 lds[LdsSize<NT>::value];
+  __shared__ c32 ub[NT];
+  __shared__ float sred[TPB / 64];
+  constexpr int E = NT / TPB;
+  const int tid = threadIdx.x;
+  const size_t row = blockIdx.x;
+  const c32* tw = reinterpret_cast<const c32*>(t.tw);
+  c32 us[E];
+  load_row<NT, TPB>(in, row, us);
+  if (in_spectral) {
+    row_fft<float, NT, TPB, true>(us, lds, tw);
+    const float invN = 1.0f / (float)NT;
+#pragma unroll
+    for (int s = 0; s < E; ++s) us[s] = invN * us[s];
+  }
+  const double* rr = rho + row * NT;
+  const double* rs = RS ? rhos + row * NT : nullptr;
+  alpha_half<E, TPB>(us, rr, p, t);
+#pragma unroll
+  for (int s = 0; s < E; ++s) ub[tid + s * TPB] = us[s];
+  const float dz = (float)p.dz;
+  c32 acc[E], k[E];
+#pragma unroll 1
+  for (int stg = 0; stg < 4; ++stg) {
+    split_row<NT, TPB, RS>(us, rr, rs, k, nullptr, p, t, lds);
+    const float w = (stg == 0 || stg == 3) ? 1.0f : 2.0f;
+    const float cdz = (stg < 2) ? 0.5f * dz : dz;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      acc[s] = (stg == 0) ? k[s] : acc[s] + w * k[s];
+      us[s] = (stg < 3) ? ub[tid + s * TPB] + cdz * k[s] : ub[tid + s * TPB] + (dz / 6.0f) * acc[s];
+    }
+  }
+  alpha_half<E, TPB>(us, rr, p, t);
+  if (out_spectral) row_fft<float, NT, TPB, false>(us, lds, tw);
+  store_row<NT, TPB>(out, row, us);
+  if (cmax_out) {
+    float m = 0.0f;
+#pragma unroll
+    for (int s = 0; s < E; ++s) m = fmaxf(m, fmaxf(fabsf(us[s].x), fabsf(us[s].y)));
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    if ((tid & 63) == 0) sred[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < TPB / 64; ++w) m = fmaxf(m, sred[w]);
+      atomicMax(cmax_out, __float_as_uint(m));
+    }
+  }
+}
+
+}  // namespace fpnl32

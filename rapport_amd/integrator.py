@@ -42,8 +42,6 @@ class Integrator:
     def __init__(self, cfg: Config, envelope: Union[str, Callable] = "gaussian_focused",
                  device=None):
         self.cfg = cfg
-        if cfg.precision != "fp64":
-            from .mixed import MixedIntegrator  # noqa: F401  (raises if unavailable)
         if cfg.write_output:
             os.makedirs(cfg.out_dir, exist_ok=True)
         g = build_grids(cfg)
@@ -66,6 +64,11 @@ class Integrator:
         self.rho = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
         self.rho_s = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
         self.mask_r = self.t.mask_r
+        self._mx = None
+        if cfg.precision == "mixed":
+            from .mixed import MixedEngine
+            self._mx = MixedEngine(self.t, cfg, g, self._u)
+            self._u = None
 
         r = g["rlist"]
         self._r_w = 2.0 * np.pi * r * np.diff(r, prepend=0.0)   # 2 pi r dr (rectangle)
@@ -150,7 +153,10 @@ class Integrator:
     # ------------------------------------------------------------------ state
     @property
     def u(self):
-        """Time-domain field (Nr x Nt complex128 on the GPU)."""
+        """Time-domain field (Nr x Nt complex128 on the GPU; the mixed path upcasts
+        its complex64 state)."""
+        if self._mx is not None:
+            return self._mx.time_c64().to(torch.complex128)
         if self._spec is not None:
             self._u = torch.empty_like(self._spec)
             call("fp_fft_rows", ptr(self._spec), ptr(self._u), self.Nr, self.Nt, 1, ptr(self.t.tw),
@@ -160,6 +166,9 @@ class Integrator:
 
     @u.setter
     def u(self, value):
+        if self._mx is not None:
+            self._mx.set_time(value)
+            return
         self._u = value.contiguous()
         self._spec = None
 
@@ -177,6 +186,9 @@ class Integrator:
     # ------------------------------------------------------------------ one step
     def step(self, dz):
         """Integrator.step (integrator.py:138-149) on the time-domain field."""
+        if self._mx is not None:
+            self._mx.strang(dz, self.rho, self._rho_s_arg(), self.nl.params(dz))
+            return
         u = self.lin.half_linear(self.u, dz)
         call("fp_nl_step", ptr(u), ptr(u), ptr(self.rho), ptr(self._rho_s_arg()), 0, 0,
              ctypes.byref(self.nl.params(dz)), ctypes.byref(self.t.tables), stream_ptr())
@@ -185,6 +197,9 @@ class Integrator:
 
     def _pipeline_step(self, dz):
         """One propagate() iteration body (plasma + step) on the spectral state."""
+        if self._mx is not None:
+            self._mx.step(dz, self.rho, self.rho_s, self._rho_s_arg(), self.nl.params(dz))
+            return
         t = self.t
         spec = self._spectral()
         if self._w0 is None:
@@ -193,7 +208,8 @@ class Integrator:
         w0, w1 = self._w0, self._w1
         st = stream_ptr()
         pp = t.plasma_params
-        call("fp_plasma_scan_spec", ptr(spec), ptr(self.rho), ptr(self.rho_s), None,
+        rho_s = self.rho_s if self.cfg.enable_ste else None   # stays zero without STE
+        call("fp_plasma_scan_spec", ptr(spec), ptr(self.rho), ptr(rho_s), None,
              ctypes.byref(pp), ptr(t.W_lut), ptr(t.Ws_lut), ptr(t.tw), ptr(t.status), st)
         phase = t.phase_half(dz)
         call("fp_hankel_gemm", ptr(t.Yt), ptr(spec), ptr(w1), self.Nr, self.Nt, t.Rk, None,
@@ -243,7 +259,10 @@ class Integrator:
     def _record(self, z_now):
         """integrator.py:179-224 with device-side reductions."""
         k, g, cfg = self.k_save, self.g, self.cfg
-        u = self.u if self._spec is None else self._time_copy()
+        if self._mx is not None:
+            u = self.u
+        else:
+            u = self.u if self._spec is None else self._time_copy()
         t = self.t
         nb = 0 if self._bin_lo is None else int(self._bin_lo.numel())
         call("fp_record_rows", ptr(u), ptr(self.rho), ptr(self.rho_s), ptr(self._rows),

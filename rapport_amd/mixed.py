@@ -1,2 +1,156 @@
-"""Mixed-precision path (complex64 field, split-FP16 Hankel GEMM, FP64 plasma scan)."""
-raise NotImplementedError("precision='mixed' is not built yet")
+"""Mixed-precision z-step engine (``Config.precision = "mixed"``).
+
+Same split-step algorithm and kernel sequence as the FP64 pipeline of
+``integrator.py``, with the precision of each part chosen for MI355X:
+
+* field: complex64, kept in the spectral domain between steps;
+* Hankel transform: split-FP16 GEMM on MFMA (``Y = (Yh + Yl)/sY``,
+  ``B = (Bh + Bl)/sB``, products ``Yh Bh + Yh Bl + Yl Bh``) with FP32 accumulation;
+  the power-of-two scale of B comes from max|B|, left in a device word by the
+  kernel that produced B (no host synchronisation);
+* propagator phase: evaluated in FP64, reduced mod 2 pi, stored as complex64;
+* FFTs, Kerr / Raman / photo-ionisation / plasma terms and RK4: FP32;
+* carrier densities: FP64 multiple-shooting scan (the rates span many decades).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_ptr
+
+
+def _scale_exp(mx):
+    """Power-of-two exponent s with mx * 2^s < 2^13 (as the GEMM kernel does for B)."""
+    if not (mx > 0) or not np.isfinite(mx):
+        return 0
+    e = np.frexp(mx)[1]
+    return int(13 - e)
+
+
+def split_hankel_fp16(Y):
+    """Split Y (float64, M x K) into FP16 hi/lo at scale 2^s, in the GEMM's fragment
+    order [K/32][M/16][4][16][8]: element j of lane (kg, r) is Y[16 mb + r][32 ks + 8 kg + j]."""
+    M, K = Y.shape
+    if M % 128 or K % 32:
+        raise ValueError(f"split-FP16 Hankel GEMM needs Nr % 128 == 0, got {M}")
+    se = _scale_exp(float(np.abs(Y).max()))
+    Ys = Y * 2.0 ** se
+    Yh = Ys.astype(np.float16)
+    Yl = (Ys - Yh.astype(np.float64)).astype(np.float16)
+
+    def frag(A):
+        return np.ascontiguousarray(A.reshape(M // 16, 16, K // 32, 4, 8).transpose(2, 0, 3, 1, 4))
+    return frag(Yh), frag(Yl), se
+
+
+class MixedTables:
+    """Float32 / FP16 device tables of the mixed path, derived from the FP64 ones."""
+
+    def __init__(self, t64, cfg, g):
+        dev = t64.device
+        self.t64 = t64
+        self.Nr, self.Nt = t64.Nr, t64.Nt
+        Nt = self.Nt
+        c64 = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.complex64))).to(dev)  # noqa: E731
+        f32 = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32))).to(dev)     # noqa: E731
+        self.tw = c64(np.exp(-2j * np.pi * np.arange(Nt) / Nt))
+        T, iU = g["T_op"], g["inv_U_nl"]
+        self.mA, self.mB, self.mC = f32(T * T * iU), f32(-T * iU), f32(iU)
+        self.Rf = c64(g["R_f"])
+        sp = g["keldysh"]["f_spline"]
+        x = sp.x
+        h = np.diff(x)
+        pp = t64.nl_params(0.0).photo_pref       # same prefactor as the FP64 path
+        cs = np.stack([sp.c[0] * h ** 3, sp.c[1] * h ** 2, sp.c[2] * h, sp.c[3]]) * pp
+        self.knots = f32(x)
+        self.inv_h = f32(1.0 / h)
+        self.pcoef_s = f32(cs)
+        yh, yl, se = split_hankel_fp16(g["Y"])
+        self.Yh = torch.from_numpy(yh).to(dev)
+        self.Yl = torch.from_numpy(yl).to(dev)
+        self.y_scale_exp = se
+        self.mask_r = f32(g["mask_r_1d"])
+        self.Rk, self.iRk = float(t64.Rk), float(t64.iRk)
+        self._phase = {}
+        self.tables = _lib.NLTablesF32(ptr(self.tw), ptr(self.mA), ptr(self.mB), ptr(self.mC),
+                                       ptr(self.Rf), ptr(self.knots), ptr(self.inv_h),
+                                       ptr(self.pcoef_s))
+
+    def phase_half(self, dz):
+        """exp(i phi dz/2) * spec_mask with phi evaluated in FP64 and reduced mod 2 pi
+        before rounding to complex64."""
+        key = float(dz)
+        if key not in self._phase:
+            g = self.t64.g
+            rho2 = (g["rholist"] ** 2)[:, None]
+            ph = (g["delta_k"][None, :] - rho2 / (2 * g["komega"]) * g["inv_U_op"][None, :]) * dz / 2
+            ph = np.remainder(ph, 2 * np.pi)
+            v = (np.exp(1j * ph) * g["spec_mask"][None, :]).astype(np.complex64)
+            self._phase[key] = torch.from_numpy(v).to(self.t64.device)
+        return self._phase[key]
+
+
+class MixedEngine:
+    """Spectral-domain complex64 state and the mixed z-step kernel sequence."""
+
+    def __init__(self, t64, cfg, g, u0_c128):
+        self.cfg = cfg
+        self.t64 = t64
+        self.m = MixedTables(t64, cfg, g)
+        dev = t64.device
+        self.Nr, self.Nt = t64.Nr, t64.Nt
+        self.spec = torch.empty(self.Nr, self.Nt, dtype=torch.complex64, device=dev)
+        self.w0 = torch.empty_like(self.spec)
+        self.w1 = torch.empty_like(self.spec)
+        self.slots = torch.zeros(4, dtype=torch.int32, device=dev)
+        self.set_time(u0_c128)
+
+    def _slot(self, i):
+        return ctypes.c_void_p(self.slots.data_ptr() + 4 * i)
+
+    def set_time(self, u):
+        """Load a time-domain field (any complex dtype) into the spectral state."""
+        uc = u.to(torch.complex64).contiguous()
+        call("fp_fft_rows_c64", ptr(uc), ptr(self.spec), self.Nr, self.Nt, 0, ptr(self.m.tw),
+             stream_ptr())
+        self.slots.zero_()
+        call("fp_absmax_f32", ptr(self.spec), 2 * self.Nr * self.Nt, self._slot(0), stream_ptr())
+
+    def time_c64(self, out=None):
+        out = torch.empty_like(self.spec) if out is None else out
+        call("fp_fft_rows_c64", ptr(self.spec), ptr(out), self.Nr, self.Nt, 1, ptr(self.m.tw),
+             stream_ptr())
+        return out
+
+    def plasma(self, rho, rho_s):
+        t = self.t64
+        call("fp_plasma_scan_spec_c64", ptr(self.spec), ptr(rho),
+             ptr(rho_s) if self.cfg.enable_ste else None, None, ctypes.byref(t.plasma_params),
+             ptr(t.W_lut), ptr(t.Ws_lut), ptr(self.m.tw), self._slot(1), ptr(t.status),
+             stream_ptr())
+
+    def strang(self, dz, rho, rho_s_arg, nl_params, clear_slots=True):
+        """Strang step on the spectral state. Slots 1-3 must start at zero: the plasma
+        kernel clears them in the pipeline, otherwise they are cleared here."""
+        m, st = self.m, stream_ptr()
+        if clear_slots:
+            self.slots[1:].zero_()
+        Nr, Nt = self.Nr, self.Nt
+        ph = m.phase_half(dz)
+        g = lambda B, C, a, rs, phase, bin_, cout, reset: call(  # noqa: E731
+            "fp_hankel_gemm_mixed", ptr(m.Yh), ptr(m.Yl), m.y_scale_exp, ptr(B), ptr(C), Nr, Nt,
+            ctypes.c_float(a), ptr(rs), ptr(phase), bin_, cout, reset, st)
+        g(self.spec, self.w1, m.Rk, None, ph, self._slot(0), self._slot(1), None)
+        g(self.w1, self.w0, m.iRk, None, None, self._slot(1), None, self._slot(0))
+        call("fp_nl_step_c64", ptr(self.w0), ptr(self.w0), ptr(rho), ptr(rho_s_arg), 1, 1,
+             ctypes.byref(nl_params), ctypes.byref(m.tables), self._slot(2), st)
+        g(self.w0, self.w1, m.Rk, None, ph, self._slot(2), self._slot(3), None)
+        g(self.w1, self.spec, m.iRk, m.mask_r, None, self._slot(3), self._slot(0), None)
+
+    def step(self, dz, rho, rho_s, rho_s_arg, nl_params):
+        self.plasma(rho, rho_s)
+        self.strang(dz, rho, rho_s_arg, nl_params, clear_slots=False)

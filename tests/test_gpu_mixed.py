@@ -1,0 +1,122 @@
+"""Mixed-precision path (complex64 field, split-FP16 Hankel GEMM with FP32
+accumulation, FP32 transforms and RK4, FP64 carrier scan) against the FP64
+references. Tolerances, stated per output:
+* split-FP16 GEMM: max error <= 4e-6 of max|Y B| (FP32 accumulation over K);
+* split(): rel-L2 <= 3e-5 against the oracle;
+* whole runs: fluence rel-L2 <= 1e-4 against the FP64 GPU path (itself pinned to
+  the oracle at 1e-9); field and electron density rel-L2 reported and bounded.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import reference_numpy as O  # noqa: E402
+from tests._cases import C1, C2, cfg_kwargs, envelope_args, pulse_field, rel_l2  # noqa: E402
+
+
+def _announce(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("case", [C1, C2])
+def test_split_fp16_gemm(case):
+    import rapport_amd as R
+    from rapport_amd import _lib
+    from rapport_amd.mixed import split_hankel_fp16
+    _announce(f"gemm_mixed N={case['N']} Nt={case['Nt']}")
+    cfg = R.Config(**cfg_kwargs(case))
+    Y = R.build_grids(cfg)["Y"]
+    Nr, Nt = Y.shape[0], cfg.Nt
+    rng = np.random.default_rng(11)
+    B = (rng.standard_normal((Nr, Nt)) + 1j * rng.standard_normal((Nr, Nt))) * 3e9
+    B[: Nr // 8] *= 1e4                      # dynamic range across rows
+    Bd = _cuda(B.astype(np.complex64))
+    Cd = torch.empty_like(Bd)
+    yh, yl, se = split_hankel_fp16(Y)
+    Yh, Yl = _cuda(yh), _cuda(yl)
+    slots = torch.zeros(4, dtype=torch.int32, device="cuda")
+    sp = lambda i: ctypes.c_void_p(slots.data_ptr() + 4 * i)  # noqa: E731
+    _lib.call("fp_absmax_f32", _lib.ptr(Bd), 2 * Nr * Nt, sp(0), _lib.stream_ptr())
+    ph = np.exp(1j * rng.uniform(0, 2 * np.pi, (Nr, Nt)))
+    phd = _cuda(ph.astype(np.complex64))
+    _lib.call("fp_hankel_gemm_mixed", _lib.ptr(Yh), _lib.ptr(Yl), se, _lib.ptr(Bd), _lib.ptr(Cd),
+              Nr, Nt, ctypes.c_float(0.5), None, _lib.ptr(phd), sp(0), sp(1), sp(2), _lib.stream_ptr())
+    got = Cd.cpu().numpy().astype(np.complex128)
+    ref = 0.5 * (Y @ B.astype(np.complex64).astype(np.complex128)) * ph.astype(np.complex64)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 4e-6, err
+    # the epilogue left max|C| in slot 1 and cleared slot 2
+    s = slots.cpu().numpy().view(np.float32)
+    assert s[1] == pytest.approx(np.abs(got.view(np.float64)).max(), rel=1e-6)
+    assert slots[2].item() == 0
+
+
+def test_split_c64_vs_oracle():
+    import rapport_amd as R
+    from rapport_amd import _lib
+    from rapport_amd.mixed import MixedTables
+    from rapport_amd.operators import DeviceTables
+    _announce("split c64 C1")
+    kw = cfg_kwargs(C1)
+    cfg_p, cfg_o = R.Config(**kw), O.Config(**kw)
+    g_p, g_o = R.build_grids(cfg_p), O.build_grids(cfg_o)
+    t64 = DeviceTables(cfg_p, g_p)
+    mt = MixedTables(t64, cfg_p, g_p)
+    Nr, Nt = g_p["Y"].shape[0], cfg_p.Nt
+    u = pulse_field(Nr, Nt, g_p["dt"], g_p["invE2"], np.linspace(1e12, 7e13, Nr), seed=9)
+    rho = np.random.default_rng(3).uniform(0, 3e20, (Nr, Nt))
+    rhs_o, a_o = O.NonlinearOperator(cfg_o, g_o).split(u, rho, None)
+    ud = _cuda(u.astype(np.complex64))
+    rhs = torch.empty_like(ud)
+    al = torch.empty(Nr, Nt, dtype=torch.float32, device="cuda")
+    _lib.call("fp_nl_split_c64", _lib.ptr(ud), _lib.ptr(_cuda(rho)), None, _lib.ptr(rhs), _lib.ptr(al),
+              ctypes.byref(t64.nl_params(cfg_p.dz)), ctypes.byref(mt.tables), _lib.stream_ptr())
+    assert rel_l2(rhs.cpu().numpy(), rhs_o) < 3e-5
+    assert rel_l2(al.cpu().numpy(), a_o) < 3e-5
+
+
+def _runs(case, steps, save_stride, **over):
+    import rapport_amd as R
+    env = R.envelope_lens_noise(**envelope_args(case))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        kw = cfg_kwargs(case, save_stride=save_stride, precision=prec, **over)
+        it = R.Integrator(R.Config(**kw), envelope=env)
+        res = it.propagate(steps)
+        out[prec] = (it, res)
+    return out
+
+
+def test_mixed_c1_full_run_vs_fp64():
+    _announce("mixed vs fp64, C1 200 steps")
+    out = _runs(C1, None, 10)
+    (ip, rp), (im, rm) = out["fp64"], out["mixed"]
+    e_f = rel_l2(rm["fluence_rz"], rp["fluence_rz"])
+    e_u = rel_l2(im.u.cpu().numpy(), ip.u.cpu().numpy())
+    e_r = rel_l2(rm["rho_rz"], rp["rho_rz"])
+    print(f"[mixed C1] rel-L2 fluence {e_f:.2e} field {e_u:.2e} rho {e_r:.2e}")
+    assert e_f < 1e-4 and e_u < 1e-3 and e_r < 1e-3
+
+
+def test_mixed_hot_run_vs_fp64():
+    _announce("mixed vs fp64, hot 60 steps")
+    hot = dict(C1, nz=59, end=59 * 10e-6, energy_uJ=20.0)
+    out = _runs(hot, None, 5, w0=30e-6, R_factor=20.0)
+    (ip, rp), (im, rm) = out["fp64"], out["mixed"]
+    assert rp["rho_rz"].max() > 1e17
+    e_f = rel_l2(rm["fluence_rz"], rp["fluence_rz"])
+    e_r = rel_l2(rm["rho_rz"], rp["rho_rz"])
+    e_u = rel_l2(im.u.cpu().numpy(), ip.u.cpu().numpy())
+    print(f"[mixed hot] rel-L2 fluence {e_f:.2e} field {e_u:.2e} rho {e_r:.2e}")
+    assert e_f < 1e-4 and e_r < 1e-3 and e_u < 1e-3
