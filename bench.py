@@ -1,20 +1,24 @@
-"""Benchmark: z-steps/s of the split-step propagator at Nr x Nt = 1024 x 4096.
+"""Benchmark: z-steps/s of the split-step propagator at Nr x Nt = 1024 x 4096,
+mixed precision vs FP64, gated on fluence rel-L2 <= 1e-4.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed|both]
-                    [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision both|fp64|mixed]
+                    [--no-cpu-baseline] [--copy-calibration]
 
-Workload (synthetic input, host-generated and uploaded before timing): fused
-silica at 800 nm, Nr = 1024 radial rows (Hankel order 1025, R = 1.5 mm),
+Workload (synthetic input, generated on the host and uploaded before timing):
+fused silica at 800 nm, Nr = 1024 radial rows (Hankel order 1025, R = 1.5 mm),
 Nt = 4096 samples over a 1.6 ps window, dz = 5 um, 5 uJ / 85 fs / w0 = 100 um
 Gaussian with a f = 5 cm lens phase and 1 % seeded complex noise, STE off. A
-step is one full iteration of the propagate() loop (plasma scan + Strang step)
-with recording off; the field stays resident in HBM throughout.
+step is one iteration of the propagate() loop (carrier scan + Strang step) with
+recording off; the field stays resident in HBM.
 
-Prints ONE JSON line. `roofline` is for the dominant kernel (the FP64 Hankel
-GEMM, 4 per step), its average launch time measured with HIP events on the
-launch stream inside the timed region; `cpu_baseline` times the NumPy FP64
-oracle (a restatement of the reference algorithm) on the host for a bounded
-sample of the same workload.
+`value` is the mixed-precision z-steps/s when its fluence after `--acc-steps`
+steps (recorded every 40 steps) is within rel-L2 1e-4 of the FP64 run of the
+same input, otherwise the FP64 z-steps/s. `roofline` is for the Hankel GEMM
+(4 launches per step) of the headline path, its average launch time measured
+with HIP events on the launch stream inside the timed region; `roofline_nongemm`
+applies the HBM model of the non-GEMM passes (compulsory bytes per point) to
+the rest of the step. `cpu_baseline` times the NumPy FP64 oracle (a
+restatement of the reference algorithm) on the host for a bounded sample.
 """
 import argparse
 import json
@@ -26,8 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP64_MFMA_PEAK_TFLOPS = 78.6
-FP16_MFMA_PEAK_TFLOPS = 2500.0
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}     # FP64 MFMA; dense FP16 MFMA
+NONGEMM_BYTES_PER_PT = {"fp64": 7 * 16 + 2 * 8, "mixed": 7 * 8 + 2 * 8}
+REL_L2_GATE = 1e-4
 
 WORKLOAD = dict(nz=3999, Nt=4096, N=1025, R_factor=15.0, begin=0.0, end=3999 * 5e-6,
                 t_window=1.6e-12, energy_uJ=5.0, wavelength=800e-9, n2=3.2e-20, Ui_eV=9.0,
@@ -43,37 +48,38 @@ def _cores():
         return os.cpu_count() or 1
 
 
-def run_gpu(precision, steps, warmup):
-    import ctypes
-    import torch
+def _integrator(precision, save_stride=10 ** 9):
     import rapport_amd as R
-    from rapport_amd._lib import call, ptr, stream_ptr
+    kw = dict(WORKLOAD, save_stride=save_stride, precision=precision)
+    return R.Integrator(R.Config(**kw), envelope=R.envelope_lens_noise(**ENVELOPE))
 
-    kw = dict(WORKLOAD, save_stride=10 ** 9, precision=precision)
-    it = R.Integrator(R.Config(**kw), envelope=R.envelope_lens_noise(**ENVELOPE))
+
+def run_gpu(precision, steps, warmup):
+    """Times `steps` pipeline steps; HIP event pairs bracket every Hankel GEMM launch."""
+    import torch
+    import rapport_amd._lib as L
+    it = _integrator(precision)
     for _ in range(warmup):
         it._pipeline_step(it.dz)
     torch.cuda.synchronize()
-
-    # time the steps, with event pairs around every Hankel GEMM launch on the launch stream
-    t, Nr, Nt = it.t, it.Nr, it.Nt
     stream = torch.cuda.current_stream()
     gemm_ev = []
-    orig_call = call
+    orig = L.call
 
     def timed_call(name, *args):
-        if name == "fp_hankel_gemm":
+        if name.startswith("fp_hankel_gemm"):
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            orig_call(name, *args)
+            orig(name, *args)
             b.record(stream)
             gemm_ev.append((a, b))
         else:
-            orig_call(name, *args)
+            orig(name, *args)
 
     import rapport_amd.integrator as integ
-    integ.call = timed_call
+    import rapport_amd.mixed as mixed
+    integ.call, mixed.call = timed_call, timed_call
     try:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -82,18 +88,33 @@ def run_gpu(precision, steps, warmup):
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
     finally:
-        integ.call = orig_call
+        integ.call, mixed.call = orig, orig
     gemm_ms = sum(a.elapsed_time(b) for a, b in gemm_ev) / max(len(gemm_ev), 1)
-    status = int(t.status.item())
-    fl = (it.u.abs() ** 2).sum(dim=1).cpu().numpy()
-    return dict(elapsed=elapsed, ms_per_step=1e3 * elapsed / steps, gemm_ms=gemm_ms,
-                n_gemm=len(gemm_ev), status=status, fluence_rows=fl, Nr=Nr, Nt=Nt)
+    return dict(elapsed=elapsed, steps_per_s=steps / elapsed, ms_per_step=1e3 * elapsed / steps,
+                gemm_ms=gemm_ms, n_gemm=len(gemm_ev), capped_rows=int(it.t.status.item()),
+                Nr=it.Nr, Nt=it.Nt)
+
+
+def accuracy(acc_steps):
+    """Fluence / field / electron-density rel-L2 of the mixed run against the FP64 run
+    of the same input after acc_steps steps (fluence recorded every 40 steps)."""
+    import numpy as np
+    out = {}
+    for prec in ("fp64", "mixed"):
+        it = _integrator(prec, save_stride=40)
+        res = it.propagate(acc_steps)
+        out[prec] = (res, it.u.cpu().numpy())
+    (rp, up), (rm, um) = out["fp64"], out["mixed"]
+    rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    return dict(steps=acc_steps, fluence_rz=rl2(rm["fluence_rz"], rp["fluence_rz"]),
+                fluence_final=rl2(rm["fluence_rz"][-1], rp["fluence_rz"][-1]),
+                field=rl2(um, up), rho_rz=rl2(rm["rho_rz"], rp["rho_rz"]),
+                Imax_Wcm2=float(rp["Imax_z"][-1]))
 
 
 def run_cpu_baseline(n_steps=2):
     """Oracle (NumPy FP64 restatement of the reference) on the host: 1 warm-up
     step then n_steps timed steps of the same workload."""
-    import numpy as np  # noqa: F401
     from oracle import reference_numpy as O
     kw = dict(WORKLOAD, save_stride=10 ** 9)
     it = O.Integrator(O.Config(**kw), envelope=O.envelope_lens_noise(**ENVELOPE))
@@ -119,12 +140,32 @@ def copy_calibration(nbytes=256 * 2 ** 20, reps=5):
     print(json.dumps(dict(copy_bytes=nbytes, reps=reps)), flush=True)
 
 
+def _roofline(prec, r):
+    Nr, Nt = r["Nr"], r["Nt"]
+    flops = 4.0 * Nr * Nr * Nt                          # real Y (Nr x Nr) @ complex (Nr x Nt)
+    ach = flops / (r["gemm_ms"] * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[prec]
+    roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                traffic=None, kernel="hankel_gemm_f64" if prec == "fp64" else "hankel_gemm_f16x3",
+                flops_per_launch=flops, avg_launch_ms=r["gemm_ms"], launches_timed=r["n_gemm"])
+    if prec == "mixed":
+        roof["executed_flops_per_launch"] = 3 * flops
+        roof["executed_frac"] = 3 * ach / peak
+    rest_ms = r["ms_per_step"] - 4 * r["gemm_ms"]
+    nb = NONGEMM_BYTES_PER_PT[prec] * Nr * Nt
+    ng = dict(bound="hbm", bytes_per_step=nb, ms_per_step=rest_ms,
+              achieved=nb / (rest_ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s")
+    ng["frac"] = ng["achieved"] / HBM_PEAK_GBS
+    return roof, ng
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--acc-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
@@ -139,24 +180,31 @@ def main():
     if os.environ.get("RANK", "0") != "0":
         return
 
-    r = run_gpu("fp64", args.steps, args.warmup)
-    Nr, Nt = r["Nr"], r["Nt"]
-    gemm_flops = 4.0 * Nr * Nr * Nt                     # real Y (Nr x Nr) @ complex (Nr x Nt)
-    achieved = gemm_flops / (r["gemm_ms"] * 1e-3) / 1e12
-    value = args.steps / r["elapsed"]
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {p: run_gpu(p, args.steps, args.warmup) for p in precs}
+    acc = accuracy(args.acc_steps) if args.precision == "both" else None
+    head = "mixed" if (acc is not None and acc["fluence_rz"] <= REL_L2_GATE) or precs == ["mixed"] \
+        else "fp64"
+    r = runs[head]
+    roof, ng = _roofline(head, r)
     out = dict(
-        metric="z-steps/s at Nr x Nt = 1024 x 4096 (FP64 drop-in)",
-        value=value, unit="z-steps/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        metric="z-steps/s at Nr x Nt = 1024 x 4096 (mixed vs FP64) at fluence rel-L2 <= 1e-4",
+        value=r["steps_per_s"], unit="z-steps/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=r["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f64", data="synthetic",
+        dtype=("c64 field / fp16x3 MFMA fp32-accum / f64 plasma" if head == "mixed" else "f64"),
+        data="synthetic",
         config=dict(workload="fused-silica split-step propagator, 1024 radial rows x 4096 t-samples, "
                              "dz 5 um, 5 uJ 85 fs w0 100 um, lens f 5 cm, 1% seeded noise, STE off",
-                    Nr=Nr, Nt=Nt, precision="fp64", recording="off"),
-        roofline=dict(bound="mfma", achieved=achieved, peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
-                      frac=achieved / FP64_MFMA_PEAK_TFLOPS, traffic=None,
-                      kernel="hankel_gemm_f64", flops_per_launch=gemm_flops,
-                      avg_launch_ms=r["gemm_ms"], launches_timed=r["n_gemm"]),
-        plasma_scan_capped_rows=r["status"],
+                    Nr=r["Nr"], Nt=r["Nt"], precision=head, recording="off"),
+        roofline=roof, roofline_nongemm=ng,
+        fp64=dict(value=runs["fp64"]["steps_per_s"], ms_per_step=runs["fp64"]["ms_per_step"],
+                  gemm_ms=runs["fp64"]["gemm_ms"]) if "fp64" in runs else None,
+        mixed=dict(value=runs["mixed"]["steps_per_s"], ms_per_step=runs["mixed"]["ms_per_step"],
+                   gemm_ms=runs["mixed"]["gemm_ms"]) if "mixed" in runs else None,
+        mixed_speedup_vs_fp64=(runs["mixed"]["steps_per_s"] / runs["fp64"]["steps_per_s"]
+                               if len(runs) == 2 else None),
+        accuracy_mixed_vs_fp64=acc,
+        plasma_scan_capped_rows=sum(x["capped_rows"] for x in runs.values()),
     )
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline()

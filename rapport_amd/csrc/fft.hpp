@@ -18,8 +18,12 @@
 
 namespace fpfft {
 
-FP_DEV constexpr int lds_pad(int i) { return i + (i >> 4); }
-template <int N> struct LdsSize { static constexpr int value = N + N / 16; };
+// LDS slot of element i of the exchange buffer: an XOR swizzle of the low five
+// bits by bits 3..7 (a bijection on [0, N) for N >= 256) that makes every
+// Stockham write and stride-layout read of a wave free of bank conflicts for
+// 8-byte elements (and 2-way, the minimum, for 16-byte elements).
+FP_DEV constexpr int lds_pad(int i) { return i ^ ((i >> 3) & 31); }
+template <int N> struct LdsSize { static constexpr int value = N; };
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 constexpr int bitrev(int r, int nbits) {

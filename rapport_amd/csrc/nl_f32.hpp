@@ -17,28 +17,37 @@ using fpfft::LdsSize;
 using fpfft::row_fft;
 typedef cpx<float> c32;
 
+__device__ const float kP10f[18] = {1e0f, 1e1f, 1e2f, 1e3f, 1e4f, 1e5f, 1e6f, 1e7f, 1e8f,
+                                    1e9f, 1e10f, 1e11f, 1e12f, 1e13f, 1e14f, 1e15f, 1e16f, 1e17f};
+
 // |W(I)| * photo_pref * depl / (|u|^2 + 1e-30), the photo-ionisation loss rate.
+// The knot grid is 10^k (1 + 0.01 j), so the interval and the local coordinate
+// follow from the decade alone: v = (I / 10^k - 1) * 100, j = floor(v),
+// s = v - j. Only the four coefficients of interval k*900 + j are loaded.
 FP_DEV float photo_rate(float a2, float rho, const fp_nl_params& p, const fp_nl_tables_f32& t) {
   float x = a2 * (float)p.invE2;
   x = fminf(fmaxf(x, (float)p.Imin), (float)p.Imax);
-  const float* kn = t.knots;
   const int n = p.n_knots;
   const int e2 = (int)((__float_as_uint(x) >> 23) & 0xff) - 127;
-  int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;
+  int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;   // floor(e2 log10 2), possibly one low
   if (k > 16) k = 16;
-  while (k < 16 && x >= kn[(k + 1) * 900]) ++k;
-  while (k > 0 && x < kn[k * 900]) --k;
-  int j = (int)((x / kn[k * 900] - 1.0f) * 100.0f);
+  float pk = kP10f[k];
+  if (k < 16) {
+    const float pn = kP10f[k + 1];
+    if (x >= pn) { ++k; pk = pn; }
+  }
+  const float v = (x / pk - 1.0f) * 100.0f;
+  int j = (int)v;
   if (j < 0) j = 0;
   if (j > 899) j = 899;
   int i = k * 900 + j;
-  if (i > n - 2) i = n - 2;
-  while (i < n - 2 && kn[i + 1] <= x) ++i;
-  while (i > 0 && kn[i] > x) --i;
+  float s = v - (float)j;
+  if (i > n - 2) { i = n - 2; s = 1.0f; }
+  s = fminf(fmaxf(s, 0.0f), 1.0f);
   const int m = n - 1;
-  const float s = (x - kn[i]) * t.inv_h[i];
-  const float w = fabsf(((t.pcoef_s[i] * s + t.pcoef_s[m + i]) * s + t.pcoef_s[2 * m + i]) * s +
-                        t.pcoef_s[3 * m + i]);
+  const float c0 = t.pcoef_s[i], c1 = t.pcoef_s[m + i], c2 = t.pcoef_s[2 * m + i],
+              c3 = t.pcoef_s[3 * m + i];
+  const float w = fabsf(((c0 * s + c1) * s + c2) * s + c3);
   float depl = 1.0f - rho / (float)p.rho_max;
   depl = fminf(fmaxf(depl, 0.0f), 1.0f);
   return nan0(w) * depl / (a2 + 1e-30f);
@@ -53,8 +62,10 @@ FP_DEV float alpha_of(float a2, float rho, const fp_nl_params& p, const fp_nl_ta
   return a > 0.0f ? a : 0.0f;
 }
 
+// rho_row: the row's electron density as float (LDS in the fused step);
+// rhos_row: the STE density row in global memory (only read when RS).
 template <int NT, int TPB, bool RS>
-FP_DEV void split_row(const c32 (&us)[NT / TPB], const double* __restrict__ rho_row,
+FP_DEV void split_row(const c32 (&us)[NT / TPB], const float* rho_row,
                       const double* __restrict__ rhos_row, c32 (&k)[NT / TPB], float* al,
                       const fp_nl_params& p, const fp_nl_tables_f32& t, c32* lds) {
   constexpr int E = NT / TPB;
@@ -68,7 +79,7 @@ FP_DEV void split_row(const c32 (&us)[NT / TPB], const double* __restrict__ rho_
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
-      const float ph = photo_rate(a2, (float)rho_row[tid + s * TPB], p, t);
+      const float ph = photo_rate(a2, rho_row[tid + s * TPB], p, t);
       buf[s] = ph * us[s];
     }
     row_fft<float, NT, TPB, false>(buf, lds, tw);
@@ -105,7 +116,7 @@ FP_DEV void split_row(const c32 (&us)[NT / TPB], const double* __restrict__ rho_
     const c32 g = mk<float>((float)(-p.plasma_pref * p.pc_re), (float)(-p.plasma_pref * p.pc_im));
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      c32 e = (float)rho_row[tid + s * TPB] * (g * us[s]);
+      c32 e = rho_row[tid + s * TPB] * (g * us[s]);
       if (RS && p.ste_pref != 0.0) {
         const float q = (float)(p.ste_pref * rhos_row[tid + s * TPB]);
         e = e + mk<float>(-q * us[s].y, q * us[s].x);
@@ -121,7 +132,7 @@ FP_DEV void split_row(const c32 (&us)[NT / TPB], const double* __restrict__ rho_
   for (int s = 0; s < E; ++s) {
     const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
     float ph;
-    const float a = alpha_of(a2, (float)rho_row[tid + s * TPB], p, t, ph);
+    const float a = alpha_of(a2, rho_row[tid + s * TPB], p, t, ph);
     if (al) al[s] = a;
     c32 r = mk<float>(k[s].x * invN + a * us[s].x, k[s].y * invN + a * us[s].y);
     if (!(a2 >= 1e-30f)) r = mk<float>(0.f, 0.f);
@@ -150,12 +161,16 @@ __global__ __launch_bounds__(TPB) void split_kernel(const float* __restrict__ u,
                                                      float* __restrict__ alpha, fp_nl_params p,
                                                      fp_nl_tables_f32 t) {
   __shared__ c32 lds[LdsSize<NT>::value];
+  __shared__ float sr[NT];
   constexpr int E = NT / TPB;
   const size_t row = blockIdx.x;
   c32 us[E], k[E];
   float al[E];
   load_row<NT, TPB>(u, row, us);
-  split_row<NT, TPB, RS>(us, rho + row * NT, RS ? rhos + row * NT : nullptr, k, al, p, t, lds);
+#pragma unroll
+  for (int s = 0; s < E; ++s) sr[threadIdx.x + s * TPB] = (float)rho[row * NT + threadIdx.x + s * TPB];
+  __syncthreads();
+  split_row<NT, TPB, RS>(us, sr, RS ? rhos + row * NT : nullptr, k, al, p, t, lds);
   store_row<NT, TPB>(rhs, row, k);
   if (alpha) {
 #pragma unroll
@@ -164,14 +179,14 @@ __global__ __launch_bounds__(TPB) void split_kernel(const float* __restrict__ u,
 }
 
 template <int E, int TPB>
-FP_DEV void alpha_half(c32 (&u)[E], const double* __restrict__ rho_row, const fp_nl_params& p,
+FP_DEV void alpha_half(c32 (&u)[E], const float* rho_row, const fp_nl_params& p,
                        const fp_nl_tables_f32& t) {
   const float hdz = (float)(-0.5 * p.dz);
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const float a2 = u[s].x * u[s].x + u[s].y * u[s].y;
     float ph;
-    const float a = alpha_of(a2, (float)rho_row[threadIdx.x + s * TPB], p, t, ph);
+    const float a = alpha_of(a2, rho_row[threadIdx.x + s * TPB], p, t, ph);
     u[s] = expf(hdz * a) * u[s];
   }
 }
@@ -180,7 +195,7 @@ FP_DEV void alpha_half(c32 (&u)[E], const double* __restrict__ rho_row, const fp
 // and out when the flags are set. cmax_out (optional) receives max |re|, |im|
 // of the output rows (float bits, atomicMax) for the next split-FP16 GEMM.
 template <int NT, int TPB, bool RS>
-__global__ __launch_bounds__(TPB) void nl_step_kernel(const float* in, float* out,
+__global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void nl_step_kernel(const float* in, float* out,
                                                        const double* __restrict__ rho,
                                                        const double* __restrict__ rhos,
                                                        int in_spectral, int out_spectral,
@@ -188,20 +203,23 @@ __global__ __launch_bounds__(TPB) void nl_step_kernel(const float* in, float* ou
                                                        unsigned* __restrict__ cmax_out) {
   __shared__ c32 lds[LdsSize<NT>::value];
   __shared__ c32 ub[NT];
-  __shared__ float sred[TPB / 64];
+  __shared__ float sr[NT];
   constexpr int E = NT / TPB;
   const int tid = threadIdx.x;
   const size_t row = blockIdx.x;
   const c32* tw = reinterpret_cast<const c32*>(t.tw);
   c32 us[E];
   load_row<NT, TPB>(in, row, us);
+#pragma unroll
+  for (int s = 0; s < E; ++s) sr[tid + s * TPB] = (float)rho[row * NT + tid + s * TPB];
   if (in_spectral) {
     row_fft<float, NT, TPB, true>(us, lds, tw);
     const float invN = 1.0f / (float)NT;
 #pragma unroll
     for (int s = 0; s < E; ++s) us[s] = invN * us[s];
   }
-  const double* rr = rho + row * NT;
+  __syncthreads();   // sr complete (the inverse FFT above also ends in a barrier)
+  const float* rr = sr;
   const double* rs = RS ? rhos + row * NT : nullptr;
   alpha_half<E, TPB>(us, rr, p, t);
 #pragma unroll
@@ -227,6 +245,8 @@ __global__ __launch_bounds__(TPB) void nl_step_kernel(const float* in, float* ou
 #pragma unroll
     for (int s = 0; s < E; ++s) m = fmaxf(m, fmaxf(fabsf(us[s].x), fabsf(us[s].y)));
     for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    float* sred = reinterpret_cast<float*>(ub);   // the RK base is no longer needed
+    __syncthreads();
     if ((tid & 63) == 0) sred[tid >> 6] = m;
     __syncthreads();
     if (tid == 0) {
