@@ -58,7 +58,7 @@ def test_keldysh_validate_asserts_and_beta_values():
     assert 0.95 < v["gamma_at_3.5e13"] < 1.05
     assert abs(v["beta_mpi_limit"] - 1.0) < 1e-3
     assert abs(v["exp_alpha_ratio"] - 1.0) < 1e-3
-    # D3/ANALYSE_CRITIQUE.md:172 / SURVEY: W(3.5e13) = 6.44e32 (4) and 4.30e32 (2)
+    # D3/ANALYSE_CRITIQUE.md:172: W(3.5e13) = 6.44e32 (beta_den 4) and 4.30e32 (beta_den 2)
     assert f"{v['W_3.5e13_beta4']:.2e}" == "6.44e+32"
     assert f"{v['W_3.5e13_beta2']:.2e}" == "4.30e+32"
     # W_MPI(3.5e13) = 3.71e34 (D3/ANALYSE_CRITIQUE.md:65)
