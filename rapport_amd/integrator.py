@@ -59,16 +59,20 @@ class Integrator:
             u0 = u0.detach().cpu().numpy()
         u0 = np.ascontiguousarray(np.asarray(u0).astype(np.complex128, copy=False))
         self.Nr, self.Nt = u0.shape
-        self._u = torch.from_numpy(u0).to(dev)
-        self._spec = None                       # FFT_t of the field when the pipeline runs
+        u0d = torch.from_numpy(u0).to(dev)
+        self._spec = None                       # FFT_t of the field (persistent buffer)
         self.rho = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
         self.rho_s = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
         self.mask_r = self.t.mask_r
         self._mx = None
         if cfg.precision == "mixed":
             from .mixed import MixedEngine
-            self._mx = MixedEngine(self.t, cfg, g, self._u)
-            self._u = None
+            self._mx = MixedEngine(self.t, cfg, g, u0d)
+        else:
+            self._spec = torch.empty_like(u0d)
+            call("fp_fft_rows", ptr(u0d), ptr(self._spec), self.Nr, self.Nt, 0, ptr(self.t.tw),
+                 stream_ptr())
+        del u0d
 
         r = g["rlist"]
         self._r_w = 2.0 * np.pi * r * np.diff(r, prepend=0.0)   # 2 pi r dr (rectangle)
@@ -151,33 +155,29 @@ class Integrator:
         self._utmp = None
 
     # ------------------------------------------------------------------ state
+    # The field lives in one persistent buffer as its FFT along t (complex128, or
+    # complex64 in the mixed engine); kernels captured in a graph keep pointing at it.
     @property
     def u(self):
         """Time-domain field (Nr x Nt complex128 on the GPU; the mixed path upcasts
-        its complex64 state)."""
+        its complex64 state). A new tensor each call."""
         if self._mx is not None:
             return self._mx.time_c64().to(torch.complex128)
-        if self._spec is not None:
-            self._u = torch.empty_like(self._spec)
-            call("fp_fft_rows", ptr(self._spec), ptr(self._u), self.Nr, self.Nt, 1, ptr(self.t.tw),
-                 stream_ptr())
-            self._spec = None
-        return self._u
+        out = torch.empty_like(self._spec)
+        call("fp_fft_rows", ptr(self._spec), ptr(out), self.Nr, self.Nt, 1, ptr(self.t.tw),
+             stream_ptr())
+        return out
 
     @u.setter
     def u(self, value):
         if self._mx is not None:
             self._mx.set_time(value)
             return
-        self._u = value.contiguous()
-        self._spec = None
+        v = value.to(torch.complex128).contiguous()
+        call("fp_fft_rows", ptr(v), ptr(self._spec), self.Nr, self.Nt, 0, ptr(self.t.tw),
+             stream_ptr())
 
     def _spectral(self):
-        if self._spec is None:
-            self._spec = torch.empty_like(self._u)
-            call("fp_fft_rows", ptr(self._u), ptr(self._spec), self.Nr, self.Nt, 0, ptr(self.t.tw),
-                 stream_ptr())
-            self._u = None
         return self._spec
 
     def _rho_s_arg(self):
@@ -222,15 +222,52 @@ class Integrator:
         call("fp_hankel_gemm", ptr(t.Yt), ptr(w1), ptr(spec), self.Nr, self.Nt, t.iRk,
              ptr(t.mask_r), None, st)
 
+    def capture_step(self, steps=1):
+        """Capture `steps` z-steps (plasma scan + Strang step, as propagate() runs them)
+        into a HIP graph on a side stream and return it; graph.replay() then advances
+        the persistent state by that many steps. The capture itself does not advance
+        the state (one eager warm-up step runs first and allocates the work buffers)."""
+        snap = self._snapshot()
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            self._pipeline_step(self.dz)
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        torch.cuda.synchronize(self.device)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=side):
+            for _ in range(steps):
+                self._pipeline_step(self.dz)
+        torch.cuda.synchronize(self.device)
+        self._restore(snap)
+        return graph
+
+    def _snapshot(self):
+        st = (self._mx.spec if self._mx is not None else self._spec).clone()
+        return st, self.rho.clone(), self.rho_s.clone()
+
+    def _restore(self, snap):
+        st, r, rs = snap
+        (self._mx.spec if self._mx is not None else self._spec).copy_(st)
+        self.rho.copy_(r)
+        self.rho_s.copy_(rs)
+        if self._mx is not None:
+            self._mx.slots.zero_()
+            call("fp_absmax_f32", ptr(self._mx.spec), 2 * self.Nr * self.Nt, self._mx._slot(0),
+                 stream_ptr())
+
     def advance(self, i):
         """Body of the propagate() loop for iteration i (integrator.py:157-162)."""
         self._pipeline_step(self.dz)
         if i % self.cfg.save_stride == 0:
             self._record(self.cfg.begin + i * self.dz)
 
-    def propagate(self, n_steps=None):
+    def propagate(self, n_steps=None, graph=False):
+        """The z-loop (integrator.py:151-177): nz + 1 iterations (or n_steps). With
+        graph=True each iteration is one replay of a captured HIP graph of the step."""
         cfg = self.cfg
         n = cfg.nz + 1 if n_steps is None else int(n_steps)
+        g1 = self.capture_step(1) if graph else None
         it = range(n)
         pbar = None
         if cfg.verbose:
@@ -241,7 +278,12 @@ class Integrator:
             except ImportError:
                 pass
         for i in it:
-            self.advance(i)
+            if g1 is not None:
+                g1.replay()
+                if i % cfg.save_stride == 0:
+                    self._record(cfg.begin + i * self.dz)
+            else:
+                self.advance(i)
             if pbar is not None and i > 0 and i % cfg.ckpt_every == 0 and self.k_save > 0:
                 k = self.k_save - 1
                 flu = self.fluence_rz[k].double().cpu().numpy()
@@ -259,10 +301,7 @@ class Integrator:
     def _record(self, z_now):
         """integrator.py:179-224 with device-side reductions."""
         k, g, cfg = self.k_save, self.g, self.cfg
-        if self._mx is not None:
-            u = self.u
-        else:
-            u = self.u if self._spec is None else self._time_copy()
+        u = self.u
         t = self.t
         nb = 0 if self._bin_lo is None else int(self._bin_lo.numel())
         call("fp_record_rows", ptr(u), ptr(self.rho), ptr(self.rho_s), ptr(self._rows),
@@ -293,13 +332,6 @@ class Integrator:
         self.rho_s_onaxis_t[k] = self.rho_s[0].float()
         self.I_onaxis_t[k] = ((u[0].abs() ** 2) * invE2).float()
         self.k_save += 1
-
-    def _time_copy(self):
-        if self._utmp is None:
-            self._utmp = torch.empty_like(self._spec)
-        call("fp_fft_rows", ptr(self._spec), ptr(self._utmp), self.Nr, self.Nt, 1, ptr(self.t.tw),
-             stream_ptr())
-        return self._utmp
 
     def _cumulative_energy_fraction(self, dEdz_uJm):
         """Trapezoid cumulative integral of dE/dz over z, over U0 (integrator.py:226-236)."""

@@ -95,3 +95,25 @@ def test_step_api_matches_pipeline():
     # same arithmetic up to FFT placement; 3 hot nonlinear steps amplify the rounding
     assert rel_l2(a.u.cpu().numpy(), b.u.cpu().numpy()) < 1e-10
     assert rel_l2(a.rho.cpu().numpy(), b.rho.cpu().numpy()) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_matches_eager(precision):
+    # a captured z-step replayed 4 times, with eager work on the stream between
+    # replays, gives bit-identical state to 4 eager steps
+    import rapport_amd as R
+    _announce(f"graph replay vs eager, {precision}")
+    kw = cfg_kwargs(C1, energy_uJ=20.0, w0=30e-6, R_factor=20.0, precision=precision)
+    env = R.envelope_lens_noise(**envelope_args(C1))
+    a = R.Integrator(R.Config(**kw), envelope=env)
+    b = R.Integrator(R.Config(**kw), envelope=env)
+    g = b.capture_step(1)
+    junk = torch.zeros(1 << 20, device="cuda")
+    for i in range(4):
+        a._pipeline_step(a.dz)
+        g.replay()
+        junk.add_(1.0)
+        torch.cuda.synchronize()
+    assert torch.equal(a.u, b.u)
+    assert torch.equal(a.rho, b.rho)
+    assert float(junk[0]) == 4.0
