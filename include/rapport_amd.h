@@ -206,6 +206,68 @@ int fp_nl_step_c64(const float* in, float* out, const double* rho, const double*
                    int in_spectral, int out_spectral, const fp_nl_params* p,
                    const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream);
 
+/* ---------------------------------------------------------------- batched form
+ * No counterpart in the reference, which runs one propagation per process: many
+ * independent mixed-precision propagations ("instances", each with its own Nr x Nt,
+ * pulse and material parameters) advance together, one grouped launch per pass.
+ * The per-instance semantics are exactly those of the single-instance mixed
+ * sequence: fp_plasma_scan_spec_c64 (clears slots 1-3) -> G1 fp_hankel_gemm_mixed
+ * (spec -> w1, Rk, phase, slot 0 -> slot 1) -> G2 (w1 -> w0, iRk, slot 1, resets
+ * slot 0) -> fp_nl_step_c64 (w0 in place, spectral in/out, max -> slot 2) -> G3
+ * (w0 -> w1, Rk, phase, slot 2 -> slot 3) -> G4 (w1 -> spec, iRk, mask_r, slot 3 ->
+ * slot 0). All pointers are device pointers owned by the caller. */
+typedef struct fp_batch_instance {
+  int Nr, Nt;                 /* Nr % 64 == 0; Nt in {512, 1024, 2048, 4096} */
+  int y_scale_exp;            /* Y = (Yh + Yl) 2^-y_scale_exp */
+  int enable_ste;             /* 1: rho_s is evolved and enters the nonlinear step */
+  float Rk, iRk;              /* R^2/j_N and j_N/R^2 */
+  const void* Yh;             /* split Hankel matrix, fragment order (fp_hankel_gemm_mixed) */
+  const void* Yl;
+  float* spec;                /* Nr x Nt complex64 state: FFT_t of the field */
+  float* w0;                  /* Nr x Nt complex64 work buffers */
+  float* w1;
+  double* rho;                /* Nr x Nt carrier densities */
+  double* rho_s;              /* (required when enable_ste) */
+  const float* phase;         /* Nr x Nt complex64 half-step phase x spectral mask */
+  const float* mask_r;        /* Nr radial absorbing mask */
+  unsigned* slots;            /* 4 scale words (float bits); slot 0 = max|spec| */
+  const double* W_lut;        /* fp_plasma_scan tables */
+  const double* Ws_lut;
+  const float* tw;            /* Nt complex64 twiddles exp(-2 pi i k / Nt) */
+  int* status;                /* plasma-scan cap counter */
+  fp_plasma_params pp;
+  fp_nl_params np;
+  fp_nl_tables_f32 nt;
+} fp_batch_instance;
+
+typedef struct fp_batch_plan fp_batch_plan;   /* opaque */
+
+/* Passes of one batched z-step, in order (fp_batch_pass). */
+enum { FP_PASS_PLASMA = 0, FP_PASS_G1 = 1, FP_PASS_G2 = 2, FP_PASS_NL = 3, FP_PASS_G3 = 4,
+       FP_PASS_G4 = 5 };
+
+/* Validates the n descriptors (host array), copies them to the device and groups
+ * the instances per pass. Default launch configuration: GEMM tile 128 x 128 (64
+ * rows when Nr % 128 != 0), nonlinear kernel Nt/8 threads, one block per work item. */
+int fp_batch_plan_create(const fp_batch_instance* inst, int n, fp_batch_plan** plan);
+int fp_batch_plan_destroy(fp_batch_plan* plan);
+
+/* Launch configuration of instance i: GEMM tile bm x bn (each 64 or 128; Nr % bm == 0,
+ * 2 Nt % bn == 0) and nonlinear threads per row nl_tpb (Nt/8 or Nt/4); 0 keeps the
+ * current value. */
+int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb);
+
+/* Grid of the launches of pass kind 0 = plasma, 1 = GEMM, 2 = nonlinear: 0 = one
+ * block per row (tile); > 0 = that many persistent blocks, each looping over work items. */
+int fp_batch_set_grid(fp_batch_plan* plan, int kind, int blocks);
+
+/* One pass (FP_PASS_*) for every instance, or the whole z-step (fp_batch_step_mixed). */
+int fp_batch_pass(fp_batch_plan* plan, int pass, void* stream);
+int fp_batch_step_mixed(fp_batch_plan* plan, void* stream);
+
+/* Number of kernel launches one fp_batch_step_mixed makes (after grouping). */
+int fp_batch_launches(const fp_batch_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

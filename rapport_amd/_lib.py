@@ -45,6 +45,15 @@ class NLTablesF32(ctypes.Structure):
                 ("inv_h", P), ("pcoef_s", P)]
 
 
+class BatchInstance(ctypes.Structure):
+    """fp_batch_instance (one instance of the batched mixed-precision z-step)."""
+    _fields_ = [("Nr", c_int), ("Nt", c_int), ("y_scale_exp", c_int), ("enable_ste", c_int),
+                ("Rk", ctypes.c_float), ("iRk", ctypes.c_float), ("Yh", P), ("Yl", P),
+                ("spec", P), ("w0", P), ("w1", P), ("rho", P), ("rho_s", P), ("phase", P),
+                ("mask_r", P), ("slots", P), ("W_lut", P), ("Ws_lut", P), ("tw", P),
+                ("status", P), ("pp", PlasmaParams), ("np", NLParams), ("nt", NLTablesF32)]
+
+
 # name -> argtypes (restype is int for all but fp_last_error)
 _SIGS = {
     "fp_version": [],
@@ -64,6 +73,13 @@ _SIGS = {
     "fp_nl_split_c64": [P, P, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTablesF32), P],
     "fp_nl_step_c64": [P, P, P, P, c_int, c_int, ctypes.POINTER(NLParams),
                        ctypes.POINTER(NLTablesF32), P, P],
+    "fp_batch_plan_create": [ctypes.POINTER(BatchInstance), c_int, ctypes.POINTER(c_void_p)],
+    "fp_batch_plan_destroy": [P],
+    "fp_batch_set_config": [P, c_int, c_int, c_int, c_int],
+    "fp_batch_set_grid": [P, c_int, c_int],
+    "fp_batch_pass": [P, c_int, P],
+    "fp_batch_step_mixed": [P, P],
+    "fp_batch_launches": [P],
 }
 
 _lib = None

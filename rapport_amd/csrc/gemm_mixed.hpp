@@ -14,9 +14,9 @@
 //   r = row in block; element j of lane (kg*16 + r) is Y[16 mb + r][32 ks + 8 kg + j].
 // B is split on the fly while staging into LDS in the same fragment order.
 //
-// Tile 128 x 128 per 256-thread block (4 waves x 64 x 64), K slab 32, LDS
-// double-buffered (64 KiB), two blocks per CU. The epilogue also reduces
-// max|C| into a device word for the next GEMM in the chain.
+// Tile BM x BN per 256-thread block (2 x 2 waves, each (BM/2) x (BN/2)) with
+// BM, BN in {64, 128}; K slab 32, LDS double-buffered, two blocks per CU. The
+// epilogue also reduces max|C| into a device word for the next GEMM in the chain.
 #pragma once
 #include "common.hpp"
 
@@ -25,8 +25,7 @@ namespace fpgemm16 {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 128, BN = 128, BK = 32, NTHR = 256;
-constexpr int SLAB_HALVES = 8 * 4 * 16 * 8;   // 128 rows (or cols) x 32 k
+constexpr int BK = 32, NTHR = 256;
 
 FP_DEV float pow2f(int e) { return __int_as_float((e + 127) << 23); }
 
@@ -41,58 +40,59 @@ FP_DEV int scale_exp(float mx) {
   return se;
 }
 
-__global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f16x3(
-    const h8* __restrict__ Yh, const h8* __restrict__ Yl, int y_scale_exp,
-    const float* __restrict__ B, float* __restrict__ C, int M, int K, int Ncols, float alpha,
-    const float* __restrict__ rowscale, const float* __restrict__ phase,
-    const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out,
-    unsigned* __restrict__ reset_slot) {
-  __shared__ h8 sAh[2][SLAB_HALVES / 8], sAl[2][SLAB_HALVES / 8];
-  __shared__ h8 sBh[2][SLAB_HALVES / 8], sBl[2][SLAB_HALVES / 8];
-  __shared__ float sred[NTHR / 64];
+// One K slab of a BM-row (BN-column) tile is BM * 4 h8 fragments.
+template <int BM, int BN> struct Smem {
+  h8 Ah[2][BM * 4], Al[2][BM * 4];
+  h8 Bh[2][BN * 4], Bl[2][BN * 4];
+  float red[NTHR / 64];
+};
 
+// One BM x BN output tile (tm, tn) of C = alpha diag(rowscale) Y B (x phase).
+template <int BM, int BN>
+FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Yh,
+                      const h8* __restrict__ Yl, int y_scale_exp, const float* __restrict__ B,
+                      float* __restrict__ C, int M, int K, int Ncols, float alpha,
+                      const float* __restrict__ rowscale, const float* __restrict__ phase,
+                      const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out) {
   const int tid = threadIdx.x;
-  if (reset_slot && blockIdx.x == 0 && tid == 0) *reset_slot = 0u;
-
-  const int tilesM = M / BM, tilesN = Ncols / BN;
-  const int nb = tilesM * tilesN;
-  int lin = blockIdx.x;
-  if (nb % 8 == 0) lin = (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;   // XCD-contiguous panels
-  const int tm = lin % tilesM, tn = lin / tilesM;
   const int m0 = tm * BM, n0 = tn * BN;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  constexpr int FA = BM / 32, FB = BN / 32;       // 16-row / 16-col blocks per wave
+  constexpr int CPT = NTHR / BN;                   // threads per B column
+  constexpr int GPT = 4 / CPT;                     // k groups of 8 staged per thread
+  constexpr int APT = BM * 4 / NTHR;               // A fragments staged per thread
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
 
   const int sb_exp = scale_exp(__uint_as_float(*bmax_in));
   const float sB = pow2f(sb_exp);
 
   // staging roles
-  const int bc = tid & 127;            // B column within the tile
-  const int bkp = tid >> 7;            // k-group pair (groups 2 bkp, 2 bkp + 1)
-  float rb[16];
-  h8 ra_h[2], ra_l[2];
+  const int bc = tid % BN;             // B column within the tile
+  const int bkp = tid / BN;            // k groups GPT bkp .. GPT bkp + GPT - 1
+  float rb[8 * GPT];
+  h8 ra_h[APT], ra_l[APT];
   const size_t a_slab_stride = (size_t)(M / 16) * 64;   // h8 entries per K slab
   const size_t a_tile_off = (size_t)(m0 / 16) * 64;
 
   auto gload = [&](int ks) {
     const int k0 = ks * BK;
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const int kg = 2 * bkp + g;
+    for (int g = 0; g < GPT; ++g) {
+      const int kg = GPT * bkp + g;
 #pragma unroll
       for (int j = 0; j < 8; ++j) rb[g * 8 + j] = B[(size_t)(k0 + 8 * kg + j) * Ncols + n0 + bc];
     }
     const size_t base = (size_t)ks * a_slab_stride + a_tile_off;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < APT; ++i) {
       ra_h[i] = Yh[base + tid + i * NTHR];
       ra_l[i] = Yl[base + tid + i * NTHR];
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const int kg = 2 * bkp + g;
+    for (int g = 0; g < GPT; ++g) {
+      const int kg = GPT * bkp + g;
       h8 hi, lo;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
@@ -102,21 +102,21 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f16x3(
         lo[j] = (_Float16)(x - (float)h);
       }
       const int idx = ((bc >> 4) * 4 + kg) * 16 + (bc & 15);
-      sBh[buf][idx] = hi;
-      sBl[buf][idx] = lo;
+      sm.Bh[buf][idx] = hi;
+      sm.Bl[buf][idx] = lo;
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      sAh[buf][tid + i * NTHR] = ra_h[i];
-      sAl[buf][tid + i * NTHR] = ra_l[i];
+    for (int i = 0; i < APT; ++i) {
+      sm.Ah[buf][tid + i * NTHR] = ra_h[i];
+      sm.Al[buf][tid + i * NTHR] = ra_l[i];
     }
   };
 
-  f4 acc[4][4];
+  f4 acc[FA][FB];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < FA; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < FB; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
   const int nslab = K / BK;
   gload(0);
@@ -126,23 +126,23 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f16x3(
   for (int s = 0; s < nslab; ++s) {
     const int cur = s & 1;
     if (s + 1 < nslab) gload(s + 1);
-    h8 ah[4], al[4], bh[4], bl[4];
+    h8 ah[FA], al[FA], bh[FB], bl[FB];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < FA; ++a) {
       const int idx = (((wm >> 4) + a) * 4 + lq) * 16 + lr;
-      ah[a] = sAh[cur][idx];
-      al[a] = sAl[cur][idx];
+      ah[a] = sm.Ah[cur][idx];
+      al[a] = sm.Al[cur][idx];
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < FB; ++b) {
       const int idx = (((wn >> 4) + b) * 4 + lq) * 16 + lr;
-      bh[b] = sBh[cur][idx];
-      bl[b] = sBl[cur][idx];
+      bh[b] = sm.Bh[cur][idx];
+      bl[b] = sm.Bl[cur][idx];
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < FA; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
+      for (int b = 0; b < FB; ++b) {
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
@@ -157,13 +157,13 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f16x3(
   const float unscale = pow2f(-(y_scale_exp + sb_exp));
   float vmax = 0.0f;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
+  for (int a = 0; a < FA; ++a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int m = m0 + wm + a * 16 + 4 * lq + v;
       const float rs = rowscale ? alpha * unscale * rowscale[m] : alpha * unscale;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
+      for (int b = 0; b < FB; ++b) {
         const int n = n0 + wn + b * 16 + lr;
         float val = acc[a][b][v] * rs;
         if (phase) {
@@ -179,14 +179,34 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f16x3(
   }
   if (cmax_out) {
     for (int d = 32; d >= 1; d >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, d, 64));
-    if (lane == 0) sred[wave] = vmax;
+    if (lane == 0) sm.red[wave] = vmax;
     __syncthreads();
     if (tid == 0) {
-      float m = sred[0];
-      for (int w = 1; w < NTHR / 64; ++w) m = fmaxf(m, sred[w]);
+      float m = sm.red[0];
+      for (int w = 1; w < NTHR / 64; ++w) m = fmaxf(m, sm.red[w]);
       atomicMax(cmax_out, __float_as_uint(m));
     }
   }
+}
+
+// XCD-aware tile order: hardware places block id on XCD (id % 8); give each XCD a
+// contiguous run of linear tiles (whole column panels).
+FP_DEV int xcd_remap(int id, int nb) {
+  return (nb % 8 == 0) ? (id % 8) * (nb / 8) + id / 8 : id;
+}
+
+__global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f16x3(
+    const h8* __restrict__ Yh, const h8* __restrict__ Yl, int y_scale_exp,
+    const float* __restrict__ B, float* __restrict__ C, int M, int K, int Ncols, float alpha,
+    const float* __restrict__ rowscale, const float* __restrict__ phase,
+    const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out,
+    unsigned* __restrict__ reset_slot) {
+  __shared__ Smem<128, 128> sm;
+  if (reset_slot && blockIdx.x == 0 && threadIdx.x == 0) *reset_slot = 0u;
+  const int tilesM = M / 128, tilesN = Ncols / 128;
+  const int lin = xcd_remap(blockIdx.x, tilesM * tilesN);
+  gemm_tile<128, 128>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
+            rowscale, phase, bmax_in, cmax_out);
 }
 
 }  // namespace fpgemm16

@@ -194,19 +194,22 @@ FP_DEV void alpha_half(c32 (&u)[E], const float* rho_row, const fp_nl_params& p,
 // Fused nonlinear step (alpha/2, RK4, alpha/2) on complex64 rows; spectral in
 // and out when the flags are set. cmax_out (optional) receives max |re|, |im|
 // of the output rows (float bits, atomicMax) for the next split-FP16 GEMM.
+template <int NT> struct StepSmem {
+  c32 lds[LdsSize<NT>::value];
+  c32 ub[NT];
+  float sr[NT];
+};
+
 template <int NT, int TPB, bool RS>
-__global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void nl_step_kernel(const float* in, float* out,
-                                                       const double* __restrict__ rho,
-                                                       const double* __restrict__ rhos,
-                                                       int in_spectral, int out_spectral,
-                                                       fp_nl_params p, fp_nl_tables_f32 t,
-                                                       unsigned* __restrict__ cmax_out) {
-  __shared__ c32 lds[LdsSize<NT>::value];
-  __shared__ c32 ub[NT];
-  __shared__ float sr[NT];
+FP_DEV void nl_step_row(StepSmem<NT>& sm, size_t row, const float* in, float* out,
+                        const double* __restrict__ rho, const double* __restrict__ rhos,
+                        int in_spectral, int out_spectral, const fp_nl_params& p,
+                        const fp_nl_tables_f32& t, unsigned* __restrict__ cmax_out) {
   constexpr int E = NT / TPB;
   const int tid = threadIdx.x;
-  const size_t row = blockIdx.x;
+  c32* lds = sm.lds;
+  c32* ub = sm.ub;
+  float* sr = sm.sr;
   const c32* tw = reinterpret_cast<const c32*>(t.tw);
   c32 us[E];
   load_row<NT, TPB>(in, row, us);
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void nl_step_kernel(cons
 #pragma unroll
     for (int s = 0; s < E; ++s) us[s] = invN * us[s];
   }
-  __syncthreads();   // sr complete (the inverse FFT above also ends in a barrier)
+  __syncthreads();
   const float* rr = sr;
   const double* rs = RS ? rhos + row * NT : nullptr;
   alpha_half<E, TPB>(us, rr, p, t);
@@ -254,6 +257,18 @@ __global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void nl_step_kernel(cons
       atomicMax(cmax_out, __float_as_uint(m));
     }
   }
+}
+
+template <int NT, int TPB, bool RS>
+__global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void nl_step_kernel(const float* in, float* out,
+                                                       const double* __restrict__ rho,
+                                                       const double* __restrict__ rhos,
+                                                       int in_spectral, int out_spectral,
+                                                       fp_nl_params p, fp_nl_tables_f32 t,
+                                                       unsigned* __restrict__ cmax_out) {
+  __shared__ StepSmem<NT> sm;
+  nl_step_row<NT, TPB, RS>(sm, blockIdx.x, in, out, rho, rhos, in_spectral, out_spectral, p, t,
+                           cmax_out);
 }
 
 }  // namespace fpnl32

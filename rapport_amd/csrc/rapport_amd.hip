@@ -2,8 +2,12 @@
 // include/rapport_amd.h) and the kernel instantiations they dispatch to.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
+#include <cstddef>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/rapport_amd.h"
 #include "common.hpp"
@@ -95,22 +99,20 @@ __global__ void absmax_kernel(const float* __restrict__ x, long long n, unsigned
 // ------------------------------------------------------------------ plasma
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
+template <typename T, int NT, int TPB>
+struct PlasmaSmem {
+  static constexpr int bytes = cmax(fpplasma::PlasmaLds<NT, TPB>::bytes,
+                                    fpfft::LdsSize<NT>::value * 2 * (int)sizeof(T));
+};
+
+// One row: (optional inverse FFT of a spectral row) -> intensity -> carrier scan.
 template <typename T, int NT, int TPB, bool STE, bool SPEC>
-__global__ __launch_bounds__(TPB) void plasma_kernel(const T* __restrict__ src,
-                                                      double* __restrict__ rho,
-                                                      double* __restrict__ rhos,
-                                                      T* __restrict__ u_out,
-                                                      fp_plasma_params p,
-                                                      const double* __restrict__ Wg,
-                                                      const double* __restrict__ Ws,
-                                                      const T* __restrict__ tw, int* status,
-                                                      unsigned* clear_slots) {
-  constexpr int PL = fpplasma::PlasmaLds<NT, TPB>::bytes;
-  constexpr int FB = fpfft::LdsSize<NT>::value * 2 * sizeof(T);
-  __shared__ __attribute__((aligned(16))) char smem[cmax(PL, FB)];
+FP_DEV void plasma_row_body(char* smem, size_t row, const T* __restrict__ src,
+                            double* __restrict__ rho, double* __restrict__ rhos,
+                            T* __restrict__ u_out, const fp_plasma_params& p,
+                            const double* __restrict__ Wg, const double* __restrict__ Ws,
+                            const T* __restrict__ tw, int* status) {
   double* sI = reinterpret_cast<double*>(smem);
-  const size_t row = blockIdx.x;
-  if (clear_slots && blockIdx.x == 0 && threadIdx.x < 3) clear_slots[threadIdx.x] = 0u;
   if (SPEC) {
     constexpr int E = NT / TPB;
     cpx<T> v[E];
@@ -144,6 +146,22 @@ __global__ __launch_bounds__(TPB) void plasma_kernel(const T* __restrict__ src,
   __syncthreads();
   fpplasma::plasma_row<NT, TPB, STE>(sI, p, Wg, Ws, rho + row * NT,
                                      rhos ? rhos + row * NT : nullptr, status);
+}
+
+template <typename T, int NT, int TPB, bool STE, bool SPEC>
+__global__ __launch_bounds__(TPB) void plasma_kernel(const T* __restrict__ src,
+                                                      double* __restrict__ rho,
+                                                      double* __restrict__ rhos,
+                                                      T* __restrict__ u_out,
+                                                      fp_plasma_params p,
+                                                      const double* __restrict__ Wg,
+                                                      const double* __restrict__ Ws,
+                                                      const T* __restrict__ tw, int* status,
+                                                      unsigned* clear_slots) {
+  __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<T, NT, TPB>::bytes];
+  if (clear_slots && blockIdx.x == 0 && threadIdx.x < 3) clear_slots[threadIdx.x] = 0u;
+  plasma_row_body<T, NT, TPB, STE, SPEC>(smem, blockIdx.x, src, rho, rhos, u_out, p, Wg, Ws, tw,
+                                         status);
 }
 
 template <typename T, int NT, bool SPEC>
@@ -180,6 +198,271 @@ int dispatch_plasma(const T* src, double* rho, double* rhos, T* u_out, const fp_
 // ------------------------------------------------------------------ nonlinear
 template <int NT> struct NLT { static constexpr int TPB = NT >= 4096 ? NT / 8 : NT / 4; };
 template <int NT> struct NLT32 { static constexpr int TPB = NT / 8; };
+
+// ------------------------------------------------------------------ batched form
+// Work items (rows of the row kernels, output tiles of the GEMM) of every instance
+// of a group are numbered consecutively: instance g.idx[j] owns items
+// [g.begin[j], g.begin[j+1]). A block maps its item to (instance, local item) by a
+// binary search over the group (uniform per block, so scalar arithmetic); with a
+// persistent grid (gridDim.x < items) each block loops over items.
+constexpr int kGroupMax = 128;
+// The descriptor is shared between host and device code: same layout on both sides.
+static_assert(sizeof(fp_batch_instance) == 432 && offsetof(fp_batch_instance, pp) == 136 &&
+                  offsetof(fp_batch_instance, np) == 240 && offsetof(fp_batch_instance, nt) == 368,
+              "fp_batch_instance layout");
+struct BatchGroup {
+  int n;
+  int idx[kGroupMax];
+  int begin[kGroupMax + 1];
+};
+
+FP_DEV int group_find(const BatchGroup& g, int item) {
+  int lo = 0, hi = g.n;   // g.begin[lo] <= item < g.begin[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (g.begin[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int NT, int TPB, bool STE>
+__global__ __launch_bounds__(TPB) void batch_plasma_kernel(const fp_batch_instance* __restrict__ insts,
+                                                           BatchGroup g) {
+  __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<float, NT, TPB>::bytes];
+  const int total = g.begin[g.n];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int j = group_find(g, item);
+    const fp_batch_instance& I = insts[g.idx[j]];
+    const int row = item - g.begin[j];
+    if (row == 0 && threadIdx.x < 3) I.slots[1 + threadIdx.x] = 0u;
+    plasma_row_body<float, NT, TPB, STE, true>(smem, row, I.spec, I.rho,
+                                               STE ? I.rho_s : nullptr, nullptr, I.pp,
+                                               I.W_lut, I.Ws_lut, I.tw, I.status);
+    __syncthreads();
+  }
+}
+
+template <int NT, int TPB, bool RS>
+__global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void batch_nl_kernel(
+    const fp_batch_instance* __restrict__ insts, BatchGroup g) {
+  __shared__ fpnl32::StepSmem<NT> sm;
+  const int total = g.begin[g.n];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int j = group_find(g, item);
+    const fp_batch_instance& I = insts[g.idx[j]];
+    const int row = item - g.begin[j];
+    fpnl32::nl_step_row<NT, TPB, RS>(sm, row, I.w0, I.w0, I.rho, RS ? I.rho_s : nullptr, 1, 1,
+                                     I.np, I.nt, I.slots + 2);
+    __syncthreads();
+  }
+}
+
+// STAGE 0..3 = G1..G4 of the mixed z-step (operands and scale slots as in the
+// single-instance sequence). The stage is a template parameter: a run-time switch
+// selecting the output pointer was miscompiled by hipcc (ROCm 7.2) -- its default
+// case kept case 1's pointer.
+template <int BM, int BN, int STAGE>
+__global__ __launch_bounds__(fpgemm16::NTHR, 2) void batch_gemm_kernel(
+    const fp_batch_instance* __restrict__ insts, BatchGroup g) {
+  __shared__ fpgemm16::Smem<BM, BN> sm;
+  const int total = g.begin[g.n];
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    const int item = (gridDim.x >= total) ? fpgemm16::xcd_remap(it, total) : it;
+    const int j = group_find(g, item);
+    const fp_batch_instance& I = insts[g.idx[j]];
+    const int local = item - g.begin[j];
+    const int tilesM = I.Nr / BM;
+    //                G1        G2       G3        G4
+    const float* B = STAGE == 0 ? I.spec : STAGE == 2 ? I.w0 : I.w1;
+    float* C = STAGE == 1 ? I.w0 : STAGE == 3 ? I.spec : I.w1;
+    const float alpha = (STAGE & 1) ? I.iRk : I.Rk;
+    const float* rs = STAGE == 3 ? I.mask_r : nullptr;
+    const float* ph = (STAGE & 1) ? nullptr : I.phase;
+    const unsigned* bin = I.slots + STAGE;
+    unsigned* cout = STAGE == 0 ? I.slots + 1 : STAGE == 2 ? I.slots + 3 : STAGE == 3 ? I.slots : nullptr;
+    if (STAGE == 1 && local == 0 && threadIdx.x == 0) I.slots[0] = 0u;
+    fpgemm16::gemm_tile<BM, BN>(sm, local % tilesM, local / tilesM,
+                                reinterpret_cast<const fpgemm16::h8*>(I.Yh),
+                                reinterpret_cast<const fpgemm16::h8*>(I.Yl), I.y_scale_exp, B, C,
+                                I.Nr, I.Nr, 2 * I.Nt, alpha, rs, ph, bin, cout);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// Host-side plan of the batched form: descriptors (host copy for validation and
+// grouping, device copy read by the kernels), per-instance launch configuration
+// and the launch groups derived from it.
+struct fp_batch_plan {
+  int n = 0;
+  int device = 0;
+  std::vector<fp_batch_instance> host;
+  fp_batch_instance* dev = nullptr;
+  std::vector<int> bm, bn, nl_tpb;
+  int grid[3] = {0, 0, 0};
+  struct Launch {
+    int Nt, ste, a, b;          // row kernels: a = threads per block; GEMM: a x b tile
+    BatchGroup g;
+  };
+  std::vector<Launch> plasma, nl, gemm;
+  bool dirty = true;
+};
+
+namespace {
+
+void push_group(std::vector<fp_batch_plan::Launch>& out, const fp_batch_plan::Launch& key,
+                const std::vector<int>& members, const std::vector<int>& items) {
+  for (size_t s = 0; s < members.size(); s += kGroupMax) {
+    fp_batch_plan::Launch L = key;
+    L.g.n = 0;
+    L.g.begin[0] = 0;
+    for (size_t k = s; k < members.size() && k < s + kGroupMax; ++k) {
+      L.g.idx[L.g.n] = members[k];
+      L.g.begin[L.g.n + 1] = L.g.begin[L.g.n] + items[k];
+      ++L.g.n;
+    }
+    out.push_back(L);
+  }
+}
+
+// Groups: plasma by (Nt, STE); nonlinear by (Nt, STE, threads); GEMM by tile shape.
+// Instances keep their order inside a group.
+void rebuild_groups(fp_batch_plan& P) {
+  P.plasma.clear();
+  P.nl.clear();
+  P.gemm.clear();
+  std::vector<std::array<int, 4>> keys;
+  for (int kind = 0; kind < 3; ++kind) {
+    keys.clear();
+    for (int i = 0; i < P.n; ++i) {
+      const fp_batch_instance& I = P.host[i];
+      std::array<int, 4> k = kind == 0   ? std::array<int, 4>{I.Nt, I.enable_ste, 256, 0}
+                             : kind == 1 ? std::array<int, 4>{I.Nt, I.enable_ste, P.nl_tpb[i], 0}
+                                         : std::array<int, 4>{0, 0, P.bm[i], P.bn[i]};
+      if (std::find(keys.begin(), keys.end(), k) == keys.end()) keys.push_back(k);
+    }
+    // larger work first (GEMM: taller tiles first) so persistent tails are short
+    std::stable_sort(keys.begin(), keys.end(), [](const std::array<int, 4>& x,
+                                                  const std::array<int, 4>& y) {
+      return x[0] * 1000 + x[2] * 4 + x[3] > y[0] * 1000 + y[2] * 4 + y[3];
+    });
+    for (const auto& k : keys) {
+      std::vector<int> members, items;
+      for (int i = 0; i < P.n; ++i) {
+        const fp_batch_instance& I = P.host[i];
+        const bool match = kind == 0   ? (I.Nt == k[0] && I.enable_ste == k[1])
+                           : kind == 1 ? (I.Nt == k[0] && I.enable_ste == k[1] && P.nl_tpb[i] == k[2])
+                                       : (P.bm[i] == k[2] && P.bn[i] == k[3]);
+        if (!match) continue;
+        members.push_back(i);
+        items.push_back(kind < 2 ? I.Nr : (I.Nr / k[2]) * (2 * I.Nt / k[3]));
+      }
+      fp_batch_plan::Launch key{};
+      key.Nt = k[0];
+      key.ste = k[1];
+      key.a = k[2];
+      key.b = k[3];
+      push_group(kind == 0 ? P.plasma : kind == 1 ? P.nl : P.gemm, key, members, items);
+    }
+  }
+  P.dirty = false;
+}
+
+inline dim3 grid_of(const BatchGroup& g, int persistent) {
+  const int total = g.begin[g.n];
+  return dim3(persistent > 0 && persistent < total ? persistent : total);
+}
+
+int launch_batch_plasma(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
+  const dim3 grid = grid_of(L.g, P.grid[0]);
+#define FP_BP(NT)                                                                               \
+  if (L.ste)                                                                                    \
+    hipLaunchKernelGGL((batch_plasma_kernel<NT, 256, true>), grid, dim3(256), 0, st, P.dev, L.g); \
+  else                                                                                          \
+    hipLaunchKernelGGL((batch_plasma_kernel<NT, 256, false>), grid, dim3(256), 0, st, P.dev, L.g); \
+  break;
+  switch (L.Nt) {
+    case 512: FP_BP(512)
+    case 1024: FP_BP(1024)
+    case 2048: FP_BP(2048)
+    case 4096: FP_BP(4096)
+    default: return fail(FP_E_SHAPE, "batch plasma: bad Nt");
+  }
+#undef FP_BP
+  return check_launch("batch_plasma");
+}
+
+template <int NT, int TPB>
+void launch_batch_nl_t(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
+  const dim3 grid = grid_of(L.g, P.grid[2]);
+  if (L.ste)
+    hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, true>), grid, dim3(TPB), 0, st, P.dev, L.g);
+  else
+    hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, false>), grid, dim3(TPB), 0, st, P.dev, L.g);
+}
+
+int launch_batch_nl(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
+  const bool wide = L.a == L.Nt / 4;
+#define FP_BN(NT)                                                              \
+  if (wide) launch_batch_nl_t<NT, NT / 4>(P, L, st);                           \
+  else launch_batch_nl_t<NT, NT / 8>(P, L, st);                                \
+  break;
+  switch (L.Nt) {
+    case 512: FP_BN(512)
+    case 1024: FP_BN(1024)
+    case 2048: FP_BN(2048)
+    case 4096: FP_BN(4096)
+    default: return fail(FP_E_SHAPE, "batch nonlinear: bad Nt");
+  }
+#undef FP_BN
+  return check_launch("batch_nl_step");
+}
+
+template <int BM, int BN>
+void launch_batch_gemm_t(const fp_batch_plan& P, const fp_batch_plan::Launch& L, int stage,
+                         hipStream_t st) {
+  const dim3 grid = grid_of(L.g, P.grid[1]);
+  const dim3 blk(fpgemm16::NTHR);
+  switch (stage) {
+    case 0: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 0>), grid, blk, 0, st, P.dev, L.g); break;
+    case 1: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 1>), grid, blk, 0, st, P.dev, L.g); break;
+    case 2: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 2>), grid, blk, 0, st, P.dev, L.g); break;
+    default: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 3>), grid, blk, 0, st, P.dev, L.g); break;
+  }
+}
+
+int launch_batch_gemm(const fp_batch_plan& P, const fp_batch_plan::Launch& L, int stage,
+                      hipStream_t st) {
+  if (stage < 0 || stage > 3) return fail(FP_E_ARG, "batch GEMM: bad stage");
+  if (L.a == 128 && L.b == 128) launch_batch_gemm_t<128, 128>(P, L, stage, st);
+  else if (L.a == 128 && L.b == 64) launch_batch_gemm_t<128, 64>(P, L, stage, st);
+  else if (L.a == 64 && L.b == 128) launch_batch_gemm_t<64, 128>(P, L, stage, st);
+  else if (L.a == 64 && L.b == 64) launch_batch_gemm_t<64, 64>(P, L, stage, st);
+  else return fail(FP_E_ARG, "batch GEMM: bad tile");
+  return check_launch("batch_hankel_gemm");
+}
+
+int run_pass(fp_batch_plan& P, int pass, hipStream_t st) {
+  if (P.dirty) rebuild_groups(P);
+  int rc = FP_OK;
+  switch (pass) {
+    case FP_PASS_PLASMA:
+      for (const auto& L : P.plasma) if ((rc = launch_batch_plasma(P, L, st))) return rc;
+      return FP_OK;
+    case FP_PASS_NL:
+      for (const auto& L : P.nl) if ((rc = launch_batch_nl(P, L, st))) return rc;
+      return FP_OK;
+    case FP_PASS_G1: case FP_PASS_G2: case FP_PASS_G3: case FP_PASS_G4: {
+      const int stage = pass == FP_PASS_G1 ? 0 : pass == FP_PASS_G2 ? 1 : pass == FP_PASS_G3 ? 2 : 3;
+      for (const auto& L : P.gemm) if ((rc = launch_batch_gemm(P, L, stage, st))) return rc;
+      return FP_OK;
+    }
+    default: return fail(FP_E_ARG, "fp_batch_pass: unknown pass");
+  }
+}
+
+bool valid_nt(int Nt) { return Nt == 512 || Nt == 1024 || Nt == 2048 || Nt == 4096; }
 
 }  // namespace
 
@@ -318,9 +601,9 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
   if (!Yh || !Yl || !B || !C || !bmax_in) return fail(FP_E_ARG, "fp_hankel_gemm_mixed: null pointer");
   if (B == C) return fail(FP_E_ARG, "fp_hankel_gemm_mixed: C must not alias B");
   const int Ncols = 2 * Nt;
-  if (Nr <= 0 || Nr % fpgemm16::BM || Ncols % fpgemm16::BN)
+  if (Nr <= 0 || Nr % 128 || Ncols % 128)
     return fail(FP_E_SHAPE, "fp_hankel_gemm_mixed: needs Nr % 128 == 0 and 2 Nt % 128 == 0");
-  const int nb = (Nr / fpgemm16::BM) * (Ncols / fpgemm16::BN);
+  const int nb = (Nr / 128) * (Ncols / 128);
   hipLaunchKernelGGL(fpgemm16::hankel_gemm_f16x3, dim3(nb), dim3(fpgemm16::NTHR), 0, S(stream),
                      reinterpret_cast<const fpgemm16::h8*>(Yh),
                      reinterpret_cast<const fpgemm16::h8*>(Yl), y_scale_exp, B, C, Nr, Nr, Ncols,
@@ -374,6 +657,107 @@ int fp_nl_step_c64(const float* in, float* out, const double* rho, const double*
     default: return fail(FP_E_SHAPE, "fp_nl_step_c64: Nt must be 512, 1024, 2048 or 4096");
   }
 #undef FP_STEP32
+}
+
+
+int fp_batch_plan_create(const fp_batch_instance* inst, int n, fp_batch_plan** plan) {
+  if (!inst || !plan || n <= 0) return fail(FP_E_ARG, "fp_batch_plan_create: bad argument");
+  *plan = nullptr;
+  for (int i = 0; i < n; ++i) {
+    const fp_batch_instance& I = inst[i];
+    char msg[160];
+    if (!valid_nt(I.Nt) || I.Nr <= 0 || I.Nr % 64) {
+      snprintf(msg, sizeof msg, "fp_batch_plan_create: instance %d: Nr=%d Nt=%d (need Nr %% 64 == 0, "
+               "Nt in {512,1024,2048,4096})", i, I.Nr, I.Nt);
+      return fail(FP_E_SHAPE, msg);
+    }
+    if (I.pp.Nr != I.Nr || I.pp.Nt != I.Nt || I.np.Nr != I.Nr || I.np.Nt != I.Nt ||
+        I.pp.enable_ste != I.enable_ste || I.pp.table_size < 2) {
+      snprintf(msg, sizeof msg, "fp_batch_plan_create: instance %d: parameter blocks disagree with "
+               "its shape / STE flag", i);
+      return fail(FP_E_ARG, msg);
+    }
+    if (!I.Yh || !I.Yl || !I.spec || !I.w0 || !I.w1 || !I.rho || !I.phase || !I.mask_r ||
+        !I.slots || !I.W_lut || !I.tw || !I.nt.tw || !I.nt.mA || !I.nt.mB || !I.nt.mC ||
+        !I.nt.Rf || !I.nt.knots || !I.nt.inv_h || !I.nt.pcoef_s ||
+        (I.enable_ste && (!I.rho_s || !I.Ws_lut))) {
+      snprintf(msg, sizeof msg, "fp_batch_plan_create: instance %d: null pointer", i);
+      return fail(FP_E_ARG, msg);
+    }
+  }
+  fp_batch_plan* P = new fp_batch_plan();
+  P->n = n;
+  P->host.assign(inst, inst + n);
+  if (hipGetDevice(&P->device) != hipSuccess) {
+    delete P;
+    return fail(FP_E_LAUNCH, "fp_batch_plan_create: no current device");
+  }
+  if (hipMalloc(&P->dev, sizeof(fp_batch_instance) * n) != hipSuccess ||
+      hipMemcpy(P->dev, inst, sizeof(fp_batch_instance) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    if (P->dev) (void)hipFree(P->dev);
+    delete P;
+    return fail(FP_E_LAUNCH, "fp_batch_plan_create: device copy of the descriptors failed");
+  }
+  P->bm.resize(n);
+  P->bn.assign(n, 128);
+  P->nl_tpb.resize(n);
+  for (int i = 0; i < n; ++i) {
+    P->bm[i] = inst[i].Nr % 128 == 0 ? 128 : 64;
+    P->nl_tpb[i] = inst[i].Nt / 8;
+  }
+  *plan = P;
+  return FP_OK;
+}
+
+int fp_batch_plan_destroy(fp_batch_plan* plan) {
+  if (!plan) return FP_OK;
+  if (plan->dev) (void)hipFree(plan->dev);
+  delete plan;
+  return FP_OK;
+}
+
+int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb) {
+  if (!plan || i < 0 || i >= plan->n) return fail(FP_E_ARG, "fp_batch_set_config: bad instance");
+  const fp_batch_instance& I = plan->host[i];
+  const int m = bm ? bm : plan->bm[i], c = bn ? bn : plan->bn[i];
+  const int t = nl_tpb ? nl_tpb : plan->nl_tpb[i];
+  if ((m != 64 && m != 128) || (c != 64 && c != 128) || I.Nr % m || (2 * I.Nt) % c)
+    return fail(FP_E_SHAPE, "fp_batch_set_config: tile must be 64 or 128 and divide Nr x 2 Nt");
+  if (t != I.Nt / 8 && t != I.Nt / 4)
+    return fail(FP_E_ARG, "fp_batch_set_config: nonlinear threads must be Nt/8 or Nt/4");
+  plan->bm[i] = m;
+  plan->bn[i] = c;
+  plan->nl_tpb[i] = t;
+  plan->dirty = true;
+  return FP_OK;
+}
+
+int fp_batch_set_grid(fp_batch_plan* plan, int kind, int blocks) {
+  if (!plan || kind < 0 || kind > 2 || blocks < 0) return fail(FP_E_ARG, "fp_batch_set_grid: bad argument");
+  plan->grid[kind] = blocks;
+  return FP_OK;
+}
+
+int fp_batch_pass(fp_batch_plan* plan, int pass, void* stream) {
+  if (!plan) return fail(FP_E_ARG, "fp_batch_pass: null plan");
+  return run_pass(*plan, pass, S(stream));
+}
+
+int fp_batch_step_mixed(fp_batch_plan* plan, void* stream) {
+  if (!plan) return fail(FP_E_ARG, "fp_batch_step_mixed: null plan");
+  static const int order[6] = {FP_PASS_PLASMA, FP_PASS_G1, FP_PASS_G2, FP_PASS_NL, FP_PASS_G3,
+                               FP_PASS_G4};
+  for (int k = 0; k < 6; ++k) {
+    const int rc = run_pass(*plan, order[k], S(stream));
+    if (rc) return rc;
+  }
+  return FP_OK;
+}
+
+int fp_batch_launches(const fp_batch_plan* plan) {
+  if (!plan) return -1;
+  if (plan->dirty) rebuild_groups(const_cast<fp_batch_plan&>(*plan));
+  return (int)(plan->plasma.size() + plan->nl.size() + 4 * plan->gemm.size());
 }
 
 }  // extern "C"
