@@ -2,7 +2,7 @@
 mixed precision vs FP64, gated on fluence rel-L2 <= 1e-4.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision both|fp64|mixed]
-                    [--no-cpu-baseline] [--copy-calibration]
+                    [--no-cpu-baseline] [--batched N] [--copy-calibration]
 
 Workload (synthetic input, generated on the host and uploaded before timing):
 fused silica at 800 nm, Nr = 1024 radial rows (Hankel order 1025, R = 1.5 mm),
@@ -19,6 +19,10 @@ with HIP events on the launch stream inside the timed region; `roofline_nongemm`
 applies the HBM model of the non-GEMM passes (compulsory bytes per point) to
 the rest of the step. `cpu_baseline` times the NumPy FP64 oracle (a
 restatement of the reference algorithm) on the host for a bounded sample.
+`batched` reports the batched form on a 64-instance sweep (Nr in {256, 512,
+768, 1024}, Nt in {1024, 2048, 4096}, 0.5-20 uJ, 40-200 fs, w0 50-150 um, 1 %
+noise, dz 10 um): instance z-steps/s of the graph-captured batched step, the
+time of each pass and the HBM fraction of the non-GEMM passes.
 """
 import argparse
 import json
@@ -112,6 +116,52 @@ def accuracy(acc_steps):
                 Imax_Wcm2=float(rp["Imax_z"][-1]))
 
 
+def run_batched(n_inst=64, steps=20, seed=0, tune=True):
+    """Batched sweep of n_inst instances (Nr in {256, 512, 768, 1024}, Nt in {1024, 2048,
+    4096}, 0.5-20 uJ log-uniform, 40-200 fs, w0 50-150 um, 1 % noise, dz 10 um), tuned,
+    one graph replay per z-step; per-pass times from HIP events around single passes."""
+    import torch
+    from rapport_amd.autotune import _time
+    from rapport_amd.batch import BatchedIntegrator, random_sweep
+    from rapport_amd._lib import call, stream_ptr
+    cfgs, envs = random_sweep(n_inst, seed=seed)
+    b = BatchedIntegrator(cfgs, envs, tune=tune)
+    st = stream_ptr()
+    passes = {}
+    for name, p in (("plasma", 0), ("G1", 1), ("G2", 2), ("nl", 3), ("G3", 4), ("G4", 5)):
+        passes[name] = _time(lambda: call("fp_batch_pass", b._plan, p, st), 1, 3)
+    b.refresh_scales()
+    g = b.capture(1)
+    for _ in range(2):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    pts = b.points()
+    gemm_ms = passes["G1"] + passes["G2"] + passes["G3"] + passes["G4"]
+    ng_ms = passes["plasma"] + passes["nl"]
+    ng_bytes = NONGEMM_BYTES_PER_PT["mixed"] * pts
+    finite = all(bool(torch.isfinite(m._mx.spec).all()) for m in b.members)
+    out = dict(instances=n_inst, steps=steps, ms_per_step=1e3 * dt, inst_steps_per_s=n_inst / dt,
+               points=pts, gpts_steps_per_s=pts / dt / 1e9, launches_per_step=b.launches_per_step(),
+               pass_ms=passes,
+               roofline_nongemm=dict(bound="hbm", bytes_per_step=ng_bytes, ms_per_step=ng_ms,
+                                     achieved=ng_bytes / (ng_ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS,
+                                     unit="GB/s", frac=ng_bytes / (ng_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
+               gemm=dict(flops_per_step=b.gemm_flops(), ms_per_step=gemm_ms,
+                         achieved_tflops=b.gemm_flops() / (gemm_ms * 1e-3) / 1e12,
+                         peak_tflops=PEAK_TFLOPS["mixed"]),
+               tuned={k: v for k, v in b.config.items() if k != "timings_ms"},
+               finite=finite, capped_rows=sum(int(m.t.status.item()) for m in b.members))
+    b.close()
+    del b, g
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_cpu_baseline(n_steps=2):
     """Oracle (NumPy FP64 restatement of the reference) on the host: 1 warm-up
     step then n_steps timed steps of the same workload."""
@@ -167,6 +217,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--acc-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched", type=int, default=64,
+                    help="instances of the batched sweep (0: skip the batched measurement)")
+    ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -206,6 +259,8 @@ def main():
         accuracy_mixed_vs_fp64=acc,
         plasma_scan_capped_rows=sum(x["capped_rows"] for x in runs.values()),
     )
+    if args.batched > 0:
+        out["batched"] = run_batched(args.batched, args.batched_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)
