@@ -206,6 +206,35 @@ int fp_nl_step_c64(const float* in, float* out, const double* rho, const double*
                    int in_spectral, int out_spectral, const fp_nl_params* p,
                    const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream);
 
+/* ---------------------------------------------------------------- adjoint
+ * No counterpart in the reference (it has no gradients): the discrete adjoint of
+ * one mixed-precision z-step, STE off. Gradients use the convention
+ * dL = Re sum conj(g) du and travel between calls as FFT_t(g) ("spectral"), like
+ * the field state. With the split-FP16 GEMM applied to Y^T (and Y^T diag(mask_r))
+ * and the conjugate half-step phase, one adjoint step is
+ *   A1 = iRk (Y^T diag(mask_r) gamma) x conj(phase), g4 = Rk Y^T A1,
+ *   fp_adj_nl_step_c64 (g4 -> g1, dL/drho),
+ *   A3 = iRk (Y^T g1) x conj(phase), gamma_n = Rk Y^T A3,
+ *   fp_adj_plasma_c64 (gamma_n += FFT_t(2 invE2 dL/dI u_n)). */
+
+/* Adjoint of the nonlinear half of Integrator.step (integrator.py:142-146) at the
+ * state v1s (spectral, the field after the first half linear step; rho = the
+ * carrier densities of the step). g4s: spectral gradient w.r.t. the input of the
+ * second half linear step; g1s (may alias g4s): spectral gradient w.r.t. v1s;
+ * grho: float Nr x Nt dL/drho; scratch: 4 Nr x Nt complex64 planes; cmax_out
+ * (optional): max |re|, |im| of g1s (float bits, atomicMax). */
+int fp_adj_nl_step_c64(const float* v1s, const double* rho, const float* g4s, float* g1s,
+                       float* grho, float* scratch, const fp_nl_params* p,
+                       const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream);
+
+/* Adjoint of the carrier scan (fp_plasma_scan_spec_c64, kernels.py:36-106, STE off):
+ * us = the spectral state the scan read, rho = its result, grho = dL/drho; adds
+ * FFT_t(2 invE2 (dL/dI) u) to the spectral gradient gs; cmax_out (optional): max of
+ * the updated gs. */
+int fp_adj_plasma_c64(const float* us, const double* rho, const float* grho, float* gs,
+                      const fp_plasma_params* p, const double* W_lut, const float* tw,
+                      unsigned* cmax_out, void* stream);
+
 /* ---------------------------------------------------------------- batched form
  * No counterpart in the reference, which runs one propagation per process: many
  * independent mixed-precision propagations ("instances", each with its own Nr x Nt,

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/rapport_amd.h"
+#include "adjoint.hpp"
 #include "common.hpp"
 #include "fft.hpp"
 #include "gemm_f64.hpp"
@@ -758,6 +759,50 @@ int fp_batch_launches(const fp_batch_plan* plan) {
   if (!plan) return -1;
   if (plan->dirty) rebuild_groups(const_cast<fp_batch_plan&>(*plan));
   return (int)(plan->plasma.size() + plan->nl.size() + 4 * plan->gemm.size());
+}
+
+
+int fp_adj_nl_step_c64(const float* v1s, const double* rho, const float* g4s, float* g1s,
+                       float* grho, float* scratch, const fp_nl_params* p,
+                       const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream) {
+  if (!v1s || !rho || !g4s || !g1s || !grho || !scratch || !p || !t)
+    return fail(FP_E_ARG, "fp_adj_nl_step_c64: null pointer");
+  if (p->Nr <= 0) return fail(FP_E_ARG, "fp_adj_nl_step_c64: bad Nr");
+  hipStream_t st = S(stream);
+#define FP_ADJNL(NT)                                                                           \
+  hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 4>), dim3(p->Nr), dim3(NT / 4), 0, st, v1s, \
+                     rho, g4s, g1s, grho, scratch, *p, *t, cmax_out);                          \
+  return check_launch("adj_nl_step_c64");
+  switch (p->Nt) {
+    case 512: { FP_ADJNL(512) }
+    case 1024: { FP_ADJNL(1024) }
+    case 2048: { FP_ADJNL(2048) }
+    case 4096: { FP_ADJNL(4096) }
+    default: return fail(FP_E_SHAPE, "fp_adj_nl_step_c64: Nt must be 512, 1024, 2048 or 4096");
+  }
+#undef FP_ADJNL
+}
+
+int fp_adj_plasma_c64(const float* us, const double* rho, const float* grho, float* gs,
+                      const fp_plasma_params* p, const double* W_lut, const float* tw,
+                      unsigned* cmax_out, void* stream) {
+  if (!us || !rho || !grho || !gs || !p || !W_lut || !tw)
+    return fail(FP_E_ARG, "fp_adj_plasma_c64: null pointer");
+  if (p->enable_ste) return fail(FP_E_ARG, "fp_adj_plasma_c64: the adjoint covers STE off");
+  if (p->Nr <= 0 || p->table_size < 2) return fail(FP_E_ARG, "fp_adj_plasma_c64: bad Nr/table_size");
+  hipStream_t st = S(stream);
+#define FP_ADJPL(NT)                                                                         \
+  hipLaunchKernelGGL((fpadj::adj_plasma_kernel<NT, 256>), dim3(p->Nr), dim3(256), 0, st, us, \
+                     rho, grho, gs, *p, W_lut, tw, cmax_out);                                \
+  return check_launch("adj_plasma_c64");
+  switch (p->Nt) {
+    case 512: { FP_ADJPL(512) }
+    case 1024: { FP_ADJPL(1024) }
+    case 2048: { FP_ADJPL(2048) }
+    case 4096: { FP_ADJPL(4096) }
+    default: return fail(FP_E_SHAPE, "fp_adj_plasma_c64: Nt must be 512, 1024, 2048 or 4096");
+  }
+#undef FP_ADJPL
 }
 
 }  // extern "C"

@@ -1,0 +1,182 @@
+"""Adjoint of the mixed-precision z-step against the FP64 oracle adjoint
+(oracle/adjoint_numpy.py, itself pinned by finite differences on the CPU).
+
+Case: Nr x Nt = 128 x 512, w0 = 30 um, 15 uJ, dz = 3 um -- Kerr + Raman,
+photo-ionisation loss and plasma all active (rho > 1e14 cm^-3). Bars, FP32 kernels
+against FP64 (stated per output):
+* nonlinear-step adjoint (g4 -> g1, dL/drho): rel-L2 <= 1e-5;
+* carrier-scan adjoint: rel-L2 <= 1e-5;
+* one whole adjoint step (GEMMs + both kernels): rel-L2 <= 2e-5;
+* gradient of the fluence loss w.r.t. 16 Laguerre-Gauss coefficients after 8
+  steps (mixed forward vs FP64 forward and adjoints): rel-L2 <= 1e-4, loss 1e-4;
+* the torch.autograd.Function gives the engine's gradient, and a short L-BFGS fit
+  lowers the loss by at least 10x.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import adjoint_numpy as A  # noqa: E402
+from oracle import reference_numpy as O  # noqa: E402
+from tests._cases import MATERIAL, rel_l2  # noqa: E402
+
+
+def _announce(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _kw(nz=8):
+    return dict(MATERIAL, nz=nz, Nt=512, N=129, R_factor=6.0, begin=0.0, end=nz * 3e-6,
+                t_window=400e-15, energy_uJ=15.0, w0=30e-6, save_stride=10 ** 6, rho_t_stride=0,
+                write_output=False, verbose=False)
+
+
+@pytest.fixture(scope="module")
+def pair():
+    import rapport_amd as R
+    from rapport_amd.adjoint import AdjointEngine
+    env_o = O.envelope_lens_noise(noise=0.05, seed=7)
+    oa = A.Adjoint(O.Integrator(O.Config(**_kw()), envelope=env_o))
+    it = R.Integrator(R.Config(**_kw(), precision="mixed"),
+                      envelope=R.envelope_lens_noise(noise=0.05, seed=7))
+    return oa, AdjointEngine(it)
+
+
+def _c64(a):
+    return torch.from_numpy(np.ascontiguousarray(a).astype(np.complex64)).cuda()
+
+
+def _spec(a):
+    return np.fft.fft(a, axis=1)
+
+
+def test_adj_nl_step_kernel(pair):
+    import ctypes
+    from rapport_amd._lib import call, ptr, stream_ptr
+    oa, eng = pair
+    _announce("adjoint nonlinear step kernel vs oracle")
+    u = oa.step(oa.it.u)                       # a state after one step
+    u = u.astype(np.complex64).astype(np.complex128)
+    rho = oa.plasma(u)
+    assert rho.max() > 1e14
+    v1 = oa.lin.half_linear(u, oa.dz).astype(np.complex64).astype(np.complex128)
+    rng = np.random.default_rng(1)
+    g4 = (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)) * np.abs(v1) / np.abs(v1).max()
+    v2 = oa.alpha_half(v1, rho)
+    v3, ys = oa.rk4(v2, rho)
+    g3, gr1 = oa.alpha_half_adj(v3, rho, g4)
+    g2, gr2 = oa.rk4_adj(ys, rho, g3)
+    g1, gr3 = oa.alpha_half_adj(v1, rho, g2)
+    v1s, g4s = _c64(_spec(v1)), _c64(_spec(g4))
+    rho_d = torch.from_numpy(rho).cuda()
+    g1s = torch.empty_like(g4s)
+    grho = torch.empty(u.shape, dtype=torch.float32, device="cuda")
+    mt = eng.mx.m
+    call("fp_adj_nl_step_c64", ptr(v1s), ptr(rho_d), ptr(g4s), ptr(g1s), ptr(grho),
+         ptr(eng.scratch), ctypes.byref(eng.nlp), ctypes.byref(mt.tables), None, stream_ptr())
+    torch.cuda.synchronize()
+    e1 = rel_l2(g1s.cpu().numpy(), _spec(g1))
+    e2 = rel_l2(grho.cpu().numpy(), gr1 + gr2 + gr3)
+    _announce(f"  g1 rel-L2 {e1:.2e}, grho rel-L2 {e2:.2e}")
+    assert e1 <= 1e-5 and e2 <= 1e-5
+
+
+def test_adj_plasma_kernel(pair):
+    import ctypes
+    from rapport_amd._lib import call, ptr, stream_ptr
+    oa, eng = pair
+    _announce("adjoint carrier scan kernel vs oracle")
+    u = oa.step(oa.it.u).astype(np.complex64).astype(np.complex128)
+    rng = np.random.default_rng(2)
+    us = _c64(_spec(u))
+    rho_d = torch.zeros(u.shape, dtype=torch.float64, device="cuda")
+    it = eng.it
+    call("fp_plasma_scan_spec_c64", ptr(us), ptr(rho_d), None, None,
+         ctypes.byref(it.t.plasma_params), ptr(it.t.W_lut), ptr(it.t.Ws_lut), ptr(eng.mx.m.tw),
+         None, ptr(it.t.status), stream_ptr())
+    torch.cuda.synchronize()
+    rho = oa.plasma(u)
+    assert rel_l2(rho_d.cpu().numpy(), rho) < 1e-6 and rho.max() > 1e14
+    gr = rng.standard_normal(u.shape).astype(np.float32)
+    gs0 = (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)) * 1e-3
+    expect = gs0 + _spec(oa.plasma_adj(u, rho, gr.astype(np.float64)))
+    gs = _c64(gs0)
+    call("fp_adj_plasma_c64", ptr(us), ptr(rho_d), ptr(torch.from_numpy(gr).cuda()), ptr(gs),
+         ctypes.byref(it.t.plasma_params), ptr(it.t.W_lut), ptr(eng.mx.m.tw), None, stream_ptr())
+    torch.cuda.synchronize()
+    e = rel_l2(gs.cpu().numpy(), expect)
+    _announce(f"  rel-L2 {e:.2e}")
+    assert e <= 1e-5
+
+
+def test_adjoint_step(pair):
+    oa, eng = pair
+    _announce("one adjoint step (GEMMs + kernels) vs oracle")
+    u = oa.step(oa.it.u).astype(np.complex64).astype(np.complex128)
+    rng = np.random.default_rng(3)
+    gn = (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)) * np.abs(u) / np.abs(u).max()
+    expect = _spec(oa.step_adj(u, gn))
+    from rapport_amd._lib import call, ptr, stream_ptr
+    eng.gam.copy_(_c64(_spec(gn)))
+    eng.aslots.zero_()
+    call("fp_absmax_f32", ptr(eng.gam), 2 * eng.Nr * eng.Nt, eng._aslot(0), stream_ptr())
+    eng._adjoint_step(_c64(_spec(u)))
+    torch.cuda.synchronize()
+    e = rel_l2(eng.gam.cpu().numpy(), expect)
+    _announce(f"  rel-L2 {e:.2e}")
+    assert e <= 2e-5
+
+
+def _lg_setup(P, seed):
+    rng = np.random.default_rng(seed)
+    theta = np.concatenate([[1.0], 0.1 * rng.standard_normal(P - 1), 0.3 * rng.standard_normal(P)])
+    hidden = theta + np.concatenate([0.05 * rng.standard_normal(P), 0.2 * rng.standard_normal(P)])
+    return theta, hidden
+
+
+def test_lg_gradient_vs_oracle(pair):
+    from rapport_amd import adjoint as AD
+    oa, eng = pair
+    n, P = 8, 8
+    _announce(f"LG gradient ({2 * P} coefficients, {n} steps) vs FP64 oracle")
+    theta, hidden = _lg_setup(P, 11)
+    g, cfg = oa.g, oa.cfg
+    F_t = A.fluence(oa.run(A.lg_field(hidden, g, cfg, P), n), g)
+    L, g0 = oa.gradient(A.lg_field(theta, g, cfg, P), n, F_t)
+    grad_o = A.lg_field_grad(theta, g, cfg, P, g0)
+    dev = eng.it.device
+    G = AD.lg_modes(eng.it.g["rlist"], eng.it.cfg.w0, P).to(dev)
+    tenv = AD.time_envelope(eng.it.g, eng.it.cfg, dev)
+    w = AD.radial_weights(eng.it.g, dev)
+    th = torch.tensor(theta, dtype=torch.float64, device=dev, requires_grad=True)
+    F = AD.fluence_from_input(AD.lg_field(th, G, float(eng.it.cfg.E0), tenv), eng, n)
+    loss = AD.fluence_loss(F, torch.from_numpy(F_t).to(dev), w)
+    loss.backward()
+    eL = abs(float(loss) - L) / L
+    e = rel_l2(th.grad.cpu().numpy(), grad_o)
+    _announce(f"  loss {float(loss):.4e} vs {L:.4e} (rel {eL:.1e}); gradient rel-L2 {e:.2e}")
+    assert eL <= 1e-4
+    assert e <= 1e-4
+
+
+def test_fit_lowers_loss(pair):
+    from rapport_amd import adjoint as AD
+    oa, eng = pair
+    n, P = 8, 4
+    _announce("L-BFGS fit of 8 LG coefficients, 8 steps")
+    theta, hidden = _lg_setup(P, 12)
+    dev = eng.it.device
+    G = AD.lg_modes(eng.it.g["rlist"], eng.it.cfg.w0, P).to(dev)
+    tenv = AD.time_envelope(eng.it.g, eng.it.cfg, dev)
+    with torch.no_grad():
+        F_t = AD.fluence_from_input(
+            AD.lg_field(torch.tensor(hidden, device=dev), G, float(eng.it.cfg.E0), tenv), eng, n).clone()
+    _, hist = AD.fit_lg(eng, F_t, torch.tensor(theta), n, iters=15)
+    _announce(f"  loss {hist[0]:.3e} -> {min(hist):.3e} in {len(hist)} evaluations")
+    assert min(hist) <= 0.1 * hist[0]
