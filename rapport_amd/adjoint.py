@@ -247,3 +247,31 @@ def fit_lg(engine, F_target, theta0, n_steps, iters=20, P=None, lr=1.0, history=
         return loss
     opt.step(closure)
     return theta.detach(), hist
+
+
+def forward_fluence(integ, u0, n):
+    """F(r) after n z-steps from the time-domain input u0, on either precision path
+    (no checkpoints, no gradient): the reference for finite differences."""
+    integ.u = u0
+    for _ in range(n):
+        integ._pipeline_step(integ.dz)
+    if integ._mx is not None:
+        s = integ._mx.spec
+        dt_invE2 = float(integ.g["dt"] * integ.g["invE2"])
+        return (s.real.double() ** 2 + s.imag.double() ** 2).sum(dim=1) * (dt_invE2 / integ.Nt)
+    s = integ._spec
+    return (s.real ** 2 + s.imag ** 2).sum(dim=1) * (float(integ.g["dt"] * integ.g["invE2"]) / integ.Nt)
+
+
+def lg_problem(P=16, seed=0, amp_sigma=0.1, phase_sigma=0.3):
+    """Start point (the plain Gaussian: a_0 = 1, others 0) and hidden coefficients of a
+    fit: a_p = delta_p0 + amp_sigma xi / (1 + p), b_p = phase_sigma xi' / (1 + p),
+    xi, xi' standard normal from numpy.random.default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    damp = 1.0 / (1.0 + np.arange(P))
+    theta0 = np.zeros(2 * P)
+    theta0[0] = 1.0
+    hidden = theta0.copy()
+    hidden[:P] += amp_sigma * rng.standard_normal(P) * damp
+    hidden[P:] += phase_sigma * rng.standard_normal(P) * damp
+    return theta0, hidden
