@@ -22,7 +22,9 @@ restatement of the reference algorithm) on the host for a bounded sample.
 `batched` reports the batched form on a 64-instance sweep (Nr in {256, 512,
 768, 1024}, Nt in {1024, 2048, 4096}, 0.5-20 uJ, 40-200 fs, w0 50-150 um, 1 %
 noise, dz 10 um): instance z-steps/s of the graph-captured batched step, the
-time of each pass and the HBM fraction of the non-GEMM passes.
+time of each pass and the HBM fraction of the non-GEMM passes. `adjoint` times one
+gradient (mixed forward with checkpoints + hand-written adjoint) at 512 x 2048 x 1000
+steps and checks 4 of its 32 components against FP64 central differences.
 """
 import argparse
 import json
@@ -99,21 +101,67 @@ def run_gpu(precision, steps, warmup):
                 Nr=it.Nr, Nt=it.Nt)
 
 
-def accuracy(acc_steps):
-    """Fluence / field / electron-density rel-L2 of the mixed run against the FP64 run
-    of the same input after acc_steps steps (fluence recorded every 40 steps)."""
+def _perturbed(it, eps, seed=99):
+    """Multiply the loaded input field by (1 + eps xi), xi complex standard normal."""
     import numpy as np
-    out = {}
+    import torch
+    u = it.u
+    rng = np.random.default_rng(seed)
+    z = torch.from_numpy(rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)).to(u.device)
+    it.u = u * (1 + eps * z)
+
+
+def accuracy(acc_steps, full_steps=0, cond_eps=1e-8):
+    """Mixed vs FP64 runs of the same input, fluence recorded every 40 steps.
+    Gate: fluence / field / electron-density rel-L2 after acc_steps steps. With
+    full_steps > acc_steps both runs continue to full_steps and the per-z fluence
+    rel-L2 is reported, next to an FP64 run whose input is perturbed by
+    (1 + cond_eps xi) -- the conditioning of the problem itself (cond_eps = 1e-8 is
+    about the rounding of storing the input in complex64)."""
+    import numpy as np
+    def up(x):
+        x = np.asarray(x)
+        return x.astype(np.complex128 if np.iscomplexobj(x) else np.float64)
+
+    def rl2(a, b):
+        return float(np.linalg.norm(up(a) - up(b)) / np.linalg.norm(up(b)))
+    runs, fields = {}, {}
     for prec in ("fp64", "mixed"):
         it = _integrator(prec, save_stride=40)
         res = it.propagate(acc_steps)
-        out[prec] = (res, it.u.cpu().numpy())
-    (rp, up), (rm, um) = out["fp64"], out["mixed"]
-    rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
-    return dict(steps=acc_steps, fluence_rz=rl2(rm["fluence_rz"], rp["fluence_rz"]),
-                fluence_final=rl2(rm["fluence_rz"][-1], rp["fluence_rz"][-1]),
-                field=rl2(um, up), rho_rz=rl2(rm["rho_rz"], rp["rho_rz"]),
-                Imax_Wcm2=float(rp["Imax_z"][-1]))
+        fields[prec] = it.u.cpu().numpy()
+        if full_steps > acc_steps:
+            res = it.propagate(full_steps - acc_steps)
+        runs[prec] = res
+        del it
+    k = (acc_steps - 1) // 40 + 1                   # records inside the gate window
+    rp, rm = runs["fp64"], runs["mixed"]
+    out = dict(steps=acc_steps, fluence_rz=rl2(rm["fluence_rz"][:k], rp["fluence_rz"][:k]),
+               fluence_final=rl2(rm["fluence_rz"][k - 1], rp["fluence_rz"][k - 1]),
+               field=rl2(fields["mixed"], fields["fp64"]),
+               rho_rz=rl2(rm["rho_rz"][:k], rp["rho_rz"][:k]),
+               Imax_Wcm2=float(rp["Imax_z"][k - 1]))
+    if full_steps > acc_steps:
+        it = _integrator("fp64", save_stride=40)
+        _perturbed(it, cond_eps)
+        it.propagate(acc_steps)
+        rc = it.propagate(full_steps - acc_steps)
+        del it
+        per_m = [rl2(rm["fluence_rz"][j], rp["fluence_rz"][j]) for j in range(rp["fluence_rz"].shape[0])]
+        per_c = [rl2(rc["fluence_rz"][j], rp["fluence_rz"][j]) for j in range(rp["fluence_rz"].shape[0])]
+        within = 0
+        for j, e in enumerate(per_m):
+            if e > REL_L2_GATE:
+                break
+            within = 40 * j
+        out["full_run"] = dict(
+            steps=full_steps, fluence_rz=rl2(rm["fluence_rz"], rp["fluence_rz"]),
+            rho_rz=rl2(rm["rho_rz"], rp["rho_rz"]), last_step_within_gate=within,
+            per_step_fluence={str(40 * j): per_m[j] for j in range(0, len(per_m), 5)},
+            fp64_input_perturbed=dict(eps=cond_eps, fluence_rz=rl2(rc["fluence_rz"], rp["fluence_rz"]),
+                                      per_step_fluence={str(40 * j): per_c[j]
+                                                        for j in range(0, len(per_c), 5)}))
+    return out
 
 
 def run_batched(n_inst=64, steps=20, seed=0, tune=True):
@@ -160,6 +208,67 @@ def run_batched(n_inst=64, steps=20, seed=0, tune=True):
     del b, g
     torch.cuda.empty_cache()
     return out
+
+
+ADJ_WORKLOAD = dict(nz=999, Nt=2048, N=513, R_factor=15.0, begin=0.0, end=999 * 10e-6,
+                    energy_uJ=3.0, wavelength=800e-9, n2=3.2e-20, Ui_eV=9.0, rho_max=2.1e22,
+                    tau_c=1.27e-15, tau_r=150e-15, delta_t=85e-15, w0=100e-6, enable_ste=False,
+                    save_stride=10 ** 6, rho_t_stride=0, write_output=False, verbose=False)
+
+
+def run_adjoint(n_fd=4, h=1e-4, P=16, seed=5):
+    """Gradient of the fluence-mismatch loss w.r.t. 2P Laguerre-Gauss coefficients after
+    1000 steps at 512 x 2048 (mixed forward + hand-written adjoint, sqrt(n)
+    checkpoints), timed; checked on n_fd coefficients against central differences of
+    the FP64 forward (h = 1e-4)."""
+    import numpy as np
+    import torch
+    import rapport_amd as R
+    from rapport_amd import adjoint as AD
+    n = ADJ_WORKLOAD["nz"] + 1
+    itm = R.Integrator(R.Config(**ADJ_WORKLOAD, precision="mixed"))
+    it64 = R.Integrator(R.Config(**ADJ_WORKLOAD, precision="fp64"))
+    eng = AD.AdjointEngine(itm)
+    dev = itm.device
+    G = AD.lg_modes(itm.g["rlist"], itm.cfg.w0, P).to(dev)
+    tenv = AD.time_envelope(itm.g, itm.cfg, dev)
+    w = AD.radial_weights(itm.g, dev)
+    E0 = float(itm.cfg.E0)
+    theta0, hidden = AD.lg_problem(P, seed=seed)
+    field = lambda th: AD.lg_field(torch.as_tensor(th, device=dev), G, E0, tenv)  # noqa: E731
+    with torch.no_grad():
+        F_t = AD.forward_fluence(it64, field(hidden).to(torch.complex128), n).clone()
+    theta = theta0 + 0.5 * (hidden - theta0)
+    th = torch.tensor(theta, dtype=torch.float64, device=dev, requires_grad=True)
+    AD.fluence_from_input(field(th), eng, n).sum().backward()      # warm-up
+    th.grad = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = AD.fluence_loss(AD.fluence_from_input(field(th), eng, n), F_t, w)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = th.grad.cpu().numpy()
+
+    def L64(x):
+        with torch.no_grad():
+            return float(AD.fluence_loss(AD.forward_fluence(it64, field(x).to(torch.complex128), n), F_t, w))
+    idx = list(range(n_fd // 2)) + list(range(P, P + n_fd - n_fd // 2))
+    fd = []
+    for k in idx:
+        e = np.zeros(2 * P)
+        e[k] = h
+        fd.append((L64(theta + e) - L64(theta - e)) / (2 * h))
+    fd = np.array(fd)
+    ga = g[idx]
+    return dict(workload="512 x 2048, 1000 steps dz 10 um, 3 uJ 85 fs w0 100 um, 2P = %d LG coefficients" % (2 * P),
+                forward_s=t1 - t0, backward_s=t2 - t1, grad_per_s=1.0 / (t2 - t0),
+                zsteps_per_s_forward=n / (t1 - t0), zsteps_per_s_backward=n / (t2 - t1),
+                loss=float(loss.detach()), fd_check=dict(h=h, coefficients=idx, adjoint=ga.tolist(),
+                                                           fd_fp64=fd.tolist(),
+                                                           rel_l2=float(np.linalg.norm(ga - fd) / np.linalg.norm(fd))))
 
 
 def run_cpu_baseline(n_steps=2):
@@ -216,10 +325,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--acc-steps", type=int, default=400)
+    ap.add_argument("--full-steps", type=int, default=4000,
+                    help="continue the accuracy runs to this many steps (0: gate window only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched", type=int, default=64,
                     help="instances of the batched sweep (0: skip the batched measurement)")
     ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -235,7 +347,7 @@ def main():
 
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {p: run_gpu(p, args.steps, args.warmup) for p in precs}
-    acc = accuracy(args.acc_steps) if args.precision == "both" else None
+    acc = accuracy(args.acc_steps, args.full_steps) if args.precision == "both" else None
     head = "mixed" if (acc is not None and acc["fluence_rz"] <= REL_L2_GATE) or precs == ["mixed"] \
         else "fp64"
     r = runs[head]
@@ -261,6 +373,8 @@ def main():
     )
     if args.batched > 0:
         out["batched"] = run_batched(args.batched, args.batched_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = run_adjoint()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)

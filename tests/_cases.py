@@ -43,5 +43,9 @@ def pulse_field(Nr, Nt, dt, invE2, peaks, seed=0, width_fs=60.0):
 
 
 def rel_l2(a, b):
+    """||a - b|| / ||b|| in float64 / complex128 (float32 squares of 1e19-scale densities
+    overflow)."""
     a, b = np.asarray(a), np.asarray(b)
+    a = a.astype(np.complex128 if np.iscomplexobj(a) else np.float64)
+    b = b.astype(np.complex128 if np.iscomplexobj(b) else np.float64)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
