@@ -129,17 +129,22 @@ class BatchedIntegrator:
         call("fp_batch_step_mixed", self._plan, stream_ptr())
 
     def _snapshot(self):
-        return [m._mx.spec.clone() for m in self.members]
+        """State of every instance: the spectral field and the carrier densities (the
+        previous densities seed the carrier scan's Newton iteration)."""
+        return [(m._mx.spec.clone(), m.rho.clone(), m.rho_s.clone()) for m in self.members]
 
     def _restore(self, snap):
-        for m, s in zip(self.members, snap):
+        for m, (s, r, rs) in zip(self.members, snap):
             m._mx.spec.copy_(s)
-            m._mx.slots.zero_()
-            call("fp_absmax_f32", ptr(m._mx.spec), 2 * m.Nr * m.Nt, m._mx._slot(0), stream_ptr())
+            m.rho.copy_(r)
+            m.rho_s.copy_(rs)
+        self.refresh_scales()
 
     def refresh_scales(self):
         """Recompute each instance's slot 0 (max|spec|) after touching the state."""
-        self._restore([m._mx.spec for m in self.members])
+        for m in self.members:
+            m._mx.slots.zero_()
+            call("fp_absmax_f32", ptr(m._mx.spec), 2 * m.Nr * m.Nt, m._mx._slot(0), stream_ptr())
 
     def capture(self, steps=1):
         """HIP graph of `steps` batched z-steps; capturing does not advance the state

@@ -32,7 +32,6 @@ namespace fpadj {
 using fpfft::LdsSize;
 using fpfft::row_fft;
 using fpnl32::c32;
-using fpnl32::kP10f;
 
 // fpnl32::photo_rate with its derivatives d/d|u|^2 and d/drho. The PCHIP piece is
 // the same cubic in the local coordinate s = (x - x_i) / h_i, h_i = 10^k / 100.
@@ -41,35 +40,22 @@ FP_DEV float photo_rate_d(float a2, float rho, const fp_nl_params& p, const fp_n
   const float xr = a2 * (float)p.invE2;
   const float x = fminf(fmaxf(xr, (float)p.Imin), (float)p.Imax);
   const int n = p.n_knots;
-  const int e2 = (int)((__float_as_uint(x) >> 23) & 0xff) - 127;
-  int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;
-  if (k > 16) k = 16;
-  float pk = kP10f[k];
-  if (k < 16) {
-    const float pn = kP10f[k + 1];
-    if (x >= pn) { ++k; pk = pn; }
-  }
-  const float v = (x / pk - 1.0f) * 100.0f;
-  int j = (int)v;
-  if (j < 0) j = 0;
-  if (j > 899) j = 899;
-  int i = k * 900 + j;
-  float s = v - (float)j;
-  if (i > n - 2) { i = n - 2; s = 1.0f; }
-  s = fminf(fmaxf(s, 0.0f), 1.0f);
+  float s, ipk;
+  const int i = fpnl32::pchip_piece(x, n, s, ipk);
   const int m = n - 1;
   const float c0 = t.pcoef_s[i], c1 = t.pcoef_s[m + i], c2 = t.pcoef_s[2 * m + i],
               c3 = t.pcoef_s[3 * m + i];
   const float f = ((c0 * s + c1) * s + c2) * s + c3;
-  float dw = ((3.0f * c0 * s + 2.0f * c1) * s + c2) * (100.0f / pk);
+  float dw = ((3.0f * c0 * s + 2.0f * c1) * s + c2) * (100.0f * ipk);
   if (f < 0.0f) dw = -dw;
   if (!(xr > (float)p.Imin && xr < (float)p.Imax)) dw = 0.0f;
-  const float dl = 1.0f - rho / (float)p.rho_max;
+  const float irm = fpnl32::rcp((float)p.rho_max);
+  const float dl = 1.0f - rho * irm;
   const float depl = fminf(fmaxf(dl, 0.0f), 1.0f);
-  const float ddepl = (dl > 0.0f && dl < 1.0f) ? -1.0f / (float)p.rho_max : 0.0f;
-  const float den = a2 + 1e-30f;
-  const float q = nan0(fabsf(f)) / den;
-  d_a2 = nan0(depl * (nan0(dw) * (float)p.invE2 - q) / den);
+  const float ddepl = (dl > 0.0f && dl < 1.0f) ? -irm : 0.0f;
+  const float iden = fpnl32::rcp(a2 + 1e-30f);
+  const float q = nan0(fabsf(f)) * iden;
+  d_a2 = nan0(depl * (nan0(dw) * (float)p.invE2 - q) * iden);
   d_rho = q * ddepl;
   return q * depl;
 }

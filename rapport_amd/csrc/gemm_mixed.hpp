@@ -69,34 +69,40 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
   // staging roles
   const int bc = tid % BN;             // B column within the tile
   const int bkp = tid / BN;            // k groups GPT bkp .. GPT bkp + GPT - 1
-  float rb[8 * GPT];
-  h8 ra_h[APT], ra_l[APT];
   const size_t a_slab_stride = (size_t)(M / 16) * 64;   // h8 entries per K slab
   const size_t a_tile_off = (size_t)(m0 / 16) * 64;
 
-  auto gload = [&](int ks) {
+  // Two register staging sets: the global loads of slab s + 2 are issued while
+  // slab s is multiplied, so each load has two slabs of MFMA work to land.
+  struct Stage {
+    float rb[8 * GPT];
+    h8 ra_h[APT], ra_l[APT];
+  };
+  Stage R0, R1;
+
+  auto gload = [&](int ks, Stage& R) {
     const int k0 = ks * BK;
 #pragma unroll
     for (int g = 0; g < GPT; ++g) {
       const int kg = GPT * bkp + g;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) rb[g * 8 + j] = B[(size_t)(k0 + 8 * kg + j) * Ncols + n0 + bc];
+      for (int j = 0; j < 8; ++j) R.rb[g * 8 + j] = B[(size_t)(k0 + 8 * kg + j) * Ncols + n0 + bc];
     }
     const size_t base = (size_t)ks * a_slab_stride + a_tile_off;
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
-      ra_h[i] = Yh[base + tid + i * NTHR];
-      ra_l[i] = Yl[base + tid + i * NTHR];
+      R.ra_h[i] = Yh[base + tid + i * NTHR];
+      R.ra_l[i] = Yl[base + tid + i * NTHR];
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, const Stage& R) {
 #pragma unroll
     for (int g = 0; g < GPT; ++g) {
       const int kg = GPT * bkp + g;
       h8 hi, lo;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const float x = rb[g * 8 + j] * sB;
+        const float x = R.rb[g * 8 + j] * sB;
         const _Float16 h = (_Float16)x;
         hi[j] = h;
         lo[j] = (_Float16)(x - (float)h);
@@ -107,8 +113,8 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
     }
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
-      sm.Ah[buf][tid + i * NTHR] = ra_h[i];
-      sm.Al[buf][tid + i * NTHR] = ra_l[i];
+      sm.Ah[buf][tid + i * NTHR] = R.ra_h[i];
+      sm.Al[buf][tid + i * NTHR] = R.ra_l[i];
     }
   };
 
@@ -118,37 +124,47 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
 #pragma unroll
     for (int b = 0; b < FB; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
-  const int nslab = K / BK;
-  gload(0);
-  sstore(0);
-  __syncthreads();
   const int lq = lane >> 4, lr = lane & 15;
-  for (int s = 0; s < nslab; ++s) {
-    const int cur = s & 1;
-    if (s + 1 < nslab) gload(s + 1);
-    h8 ah[FA], al[FA], bh[FB], bl[FB];
+  auto mma = [&](int cur) {
+    h8 ah[FA], al[FA];
 #pragma unroll
     for (int a = 0; a < FA; ++a) {
       const int idx = (((wm >> 4) + a) * 4 + lq) * 16 + lr;
       ah[a] = sm.Ah[cur][idx];
       al[a] = sm.Al[cur][idx];
     }
+    // B fragments per column block: keeps the live registers at A + one B pair
 #pragma unroll
     for (int b = 0; b < FB; ++b) {
       const int idx = (((wn >> 4) + b) * 4 + lq) * 16 + lr;
-      bh[b] = sm.Bh[cur][idx];
-      bl[b] = sm.Bl[cur][idx];
-    }
+      const h8 bh = sm.Bh[cur][idx];
+      const h8 bl = sm.Bl[cur][idx];
 #pragma unroll
-    for (int a = 0; a < FA; ++a)
-#pragma unroll
-      for (int b = 0; b < FB; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      for (int a = 0; a < FA; ++a) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh, acc[a][b], 0, 0, 0);
       }
-    if (s + 1 < nslab) {
-      sstore(cur ^ 1);
+    }
+  };
+
+  // Slab s lives in LDS buffer s & 1. Invariant at the top of an even step s: slab s is
+  // in buffer 0 (visible to all), R0 holds slab s + 1.
+  const int nslab = K / BK;
+  gload(0, R0);
+  sstore(0, R0);
+  if (nslab > 1) gload(1, R0);
+  __syncthreads();
+  for (int s = 0; s < nslab; s += 2) {
+    if (s + 2 < nslab) gload(s + 2, R1);
+    mma(0);
+    if (s + 1 >= nslab) break;
+    sstore(1, R0);
+    __syncthreads();
+    if (s + 3 < nslab) gload(s + 3, R0);
+    mma(1);
+    if (s + 2 < nslab) {
+      sstore(0, R1);
       __syncthreads();
     }
   }
@@ -195,17 +211,22 @@ FP_DEV int xcd_remap(int id, int nb) {
   return (nb % 8 == 0) ? (id % 8) * (nb / 8) + id / 8 : id;
 }
 
-__global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f16x3(
+// Blocks per CU the tile shape is built for: 128 x 256 tiles take 96 KiB of LDS
+// (one block per CU, up to 512 VGPR + AGPR per lane), the smaller ones two.
+template <int BM, int BN> struct TileOcc { static constexpr int value = (BM * BN >= 128 * 256) ? 1 : 2; };
+
+template <int BM, int BN>
+__global__ __launch_bounds__(NTHR, (TileOcc<BM, BN>::value)) void hankel_gemm_f16x3(
     const h8* __restrict__ Yh, const h8* __restrict__ Yl, int y_scale_exp,
     const float* __restrict__ B, float* __restrict__ C, int M, int K, int Ncols, float alpha,
     const float* __restrict__ rowscale, const float* __restrict__ phase,
     const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out,
     unsigned* __restrict__ reset_slot) {
-  __shared__ Smem<128, 128> sm;
+  __shared__ Smem<BM, BN> sm;
   if (reset_slot && blockIdx.x == 0 && threadIdx.x == 0) *reset_slot = 0u;
-  const int tilesM = M / 128, tilesN = Ncols / 128;
+  const int tilesM = M / BM, tilesN = Ncols / BN;
   const int lin = xcd_remap(blockIdx.x, tilesM * tilesN);
-  gemm_tile<128, 128>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
+  gemm_tile<BM, BN>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
             rowscale, phase, bmax_in, cmax_out);
 }
 

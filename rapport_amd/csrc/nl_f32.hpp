@@ -19,6 +19,30 @@ typedef cpx<float> c32;
 
 __device__ const float kP10f[18] = {1e0f, 1e1f, 1e2f, 1e3f, 1e4f, 1e5f, 1e6f, 1e7f, 1e8f,
                                     1e9f, 1e10f, 1e11f, 1e12f, 1e13f, 1e14f, 1e15f, 1e16f, 1e17f};
+__device__ const float kInvP10f[18] = {1e0f,   1e-1f,  1e-2f,  1e-3f,  1e-4f,  1e-5f,
+                                       1e-6f,  1e-7f,  1e-8f,  1e-9f,  1e-10f, 1e-11f,
+                                       1e-12f, 1e-13f, 1e-14f, 1e-15f, 1e-16f, 1e-17f};
+
+FP_DEV float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// PCHIP interval of the (clamped) intensity x on the knot grid 10^k (1 + 0.01 j):
+// returns the coefficient index i, the local coordinate s in [0, 1] and 1 / 10^k.
+FP_DEV int pchip_piece(float x, int n, float& s, float& inv_pk) {
+  const int e2 = (int)((__float_as_uint(x) >> 23) & 0xff) - 127;
+  int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;   // floor(e2 log10 2), possibly one low
+  if (k > 16) k = 16;
+  if (k < 16 && x >= kP10f[k + 1]) ++k;
+  inv_pk = kInvP10f[k];
+  const float v = (x * inv_pk - 1.0f) * 100.0f;
+  int j = (int)v;
+  if (j < 0) j = 0;
+  if (j > 899) j = 899;
+  int i = k * 900 + j;
+  s = v - (float)j;
+  if (i > n - 2) { i = n - 2; s = 1.0f; }
+  s = fminf(fmaxf(s, 0.0f), 1.0f);
+  return i;
+}
 
 // |W(I)| * photo_pref * depl / (|u|^2 + 1e-30), the photo-ionisation loss rate.
 // The knot grid is 10^k (1 + 0.01 j), so the interval and the local coordinate
@@ -28,29 +52,15 @@ FP_DEV float photo_rate(float a2, float rho, const fp_nl_params& p, const fp_nl_
   float x = a2 * (float)p.invE2;
   x = fminf(fmaxf(x, (float)p.Imin), (float)p.Imax);
   const int n = p.n_knots;
-  const int e2 = (int)((__float_as_uint(x) >> 23) & 0xff) - 127;
-  int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;   // floor(e2 log10 2), possibly one low
-  if (k > 16) k = 16;
-  float pk = kP10f[k];
-  if (k < 16) {
-    const float pn = kP10f[k + 1];
-    if (x >= pn) { ++k; pk = pn; }
-  }
-  const float v = (x / pk - 1.0f) * 100.0f;
-  int j = (int)v;
-  if (j < 0) j = 0;
-  if (j > 899) j = 899;
-  int i = k * 900 + j;
-  float s = v - (float)j;
-  if (i > n - 2) { i = n - 2; s = 1.0f; }
-  s = fminf(fmaxf(s, 0.0f), 1.0f);
+  float sl, ipk;
+  const int i = pchip_piece(x, n, sl, ipk);
   const int m = n - 1;
   const float c0 = t.pcoef_s[i], c1 = t.pcoef_s[m + i], c2 = t.pcoef_s[2 * m + i],
               c3 = t.pcoef_s[3 * m + i];
-  const float w = fabsf(((c0 * s + c1) * s + c2) * s + c3);
-  float depl = 1.0f - rho / (float)p.rho_max;
+  const float w = fabsf(((c0 * sl + c1) * sl + c2) * sl + c3);
+  float depl = 1.0f - rho * rcp((float)p.rho_max);
   depl = fminf(fmaxf(depl, 0.0f), 1.0f);
-  return nan0(w) * depl / (a2 + 1e-30f);
+  return nan0(w) * depl * rcp(a2 + 1e-30f);
 }
 
 FP_DEV float alpha_of(float a2, float rho, const fp_nl_params& p, const fp_nl_tables_f32& t,
@@ -75,19 +85,23 @@ FP_DEV void split_row(const c32 (&us)[NT / TPB], const float* rho_row,
   const c32* Rf = reinterpret_cast<const c32*>(t.Rf);
   const float invN = 1.0f / (float)NT;
   c32 buf[E];
+  float phs[E];                        // photo-ionisation rate, reused for alpha below
   if (p.en_pi_loss) {
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
-      const float ph = photo_rate(a2, rho_row[tid + s * TPB], p, t);
-      buf[s] = ph * us[s];
+      phs[s] = photo_rate(a2, rho_row[tid + s * TPB], p, t);
+      buf[s] = phs[s] * us[s];
     }
     row_fft<float, NT, TPB, false>(buf, lds, tw);
 #pragma unroll
     for (int s = 0; s < E; ++s) k[s] = t.mB[tid + s * TPB] * buf[s];
   } else {
 #pragma unroll
-    for (int s = 0; s < E; ++s) k[s] = mk<float>(0.f, 0.f);
+    for (int s = 0; s < E; ++s) {
+      k[s] = mk<float>(0.f, 0.f);
+      phs[s] = 0.0f;
+    }
   }
   if (p.en_kerr_inst || p.en_kerr_raman) {
     if (p.en_kerr_raman) {
@@ -131,8 +145,10 @@ FP_DEV void split_row(const c32 (&us)[NT / TPB], const float* rho_row,
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
-    float ph;
-    const float a = alpha_of(a2, rho_row[tid + s * TPB], p, t, ph);
+    float a = phs[s];
+    if (p.en_plasma_absorb) a = a + (float)p.plasma_pref * rho_row[tid + s * TPB];
+    a = nan0(a);
+    a = a > 0.0f ? a : 0.0f;
     if (al) al[s] = a;
     c32 r = mk<float>(k[s].x * invN + a * us[s].x, k[s].y * invN + a * us[s].y);
     if (!(a2 >= 1e-30f)) r = mk<float>(0.f, 0.f);

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void batch_nl_kernel(
 // selecting the output pointer was miscompiled by hipcc (ROCm 7.2) -- its default
 // case kept case 1's pointer.
 template <int BM, int BN, int STAGE>
-__global__ __launch_bounds__(fpgemm16::NTHR, 2) void batch_gemm_kernel(
+__global__ __launch_bounds__(fpgemm16::NTHR, (fpgemm16::TileOcc<BM, BN>::value)) void batch_gemm_kernel(
     const fp_batch_instance* __restrict__ insts, BatchGroup g) {
   __shared__ fpgemm16::Smem<BM, BN> sm;
   const int total = g.begin[g.n];
@@ -436,7 +436,8 @@ void launch_batch_gemm_t(const fp_batch_plan& P, const fp_batch_plan::Launch& L,
 int launch_batch_gemm(const fp_batch_plan& P, const fp_batch_plan::Launch& L, int stage,
                       hipStream_t st) {
   if (stage < 0 || stage > 3) return fail(FP_E_ARG, "batch GEMM: bad stage");
-  if (L.a == 128 && L.b == 128) launch_batch_gemm_t<128, 128>(P, L, stage, st);
+  if (L.a == 128 && L.b == 256) launch_batch_gemm_t<128, 256>(P, L, stage, st);
+  else if (L.a == 128 && L.b == 128) launch_batch_gemm_t<128, 128>(P, L, stage, st);
   else if (L.a == 128 && L.b == 64) launch_batch_gemm_t<128, 64>(P, L, stage, st);
   else if (L.a == 64 && L.b == 128) launch_batch_gemm_t<64, 128>(P, L, stage, st);
   else if (L.a == 64 && L.b == 64) launch_batch_gemm_t<64, 64>(P, L, stage, st);
@@ -604,10 +605,12 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
   const int Ncols = 2 * Nt;
   if (Nr <= 0 || Nr % 128 || Ncols % 128)
     return fail(FP_E_SHAPE, "fp_hankel_gemm_mixed: needs Nr % 128 == 0 and 2 Nt % 128 == 0");
-  const int nb = (Nr / 128) * (Ncols / 128);
-  hipLaunchKernelGGL(fpgemm16::hankel_gemm_f16x3, dim3(nb), dim3(fpgemm16::NTHR), 0, S(stream),
-                     reinterpret_cast<const fpgemm16::h8*>(Yh),
-                     reinterpret_cast<const fpgemm16::h8*>(Yl), y_scale_exp, B, C, Nr, Nr, Ncols,
+  // 128 x 128 tiles, two blocks per CU (a 128 x 256 tile at one block per CU measured
+  // slower at 1024 x 4096: 85 vs 66 us, latency no longer hidden).
+  const auto* yh = reinterpret_cast<const fpgemm16::h8*>(Yh);
+  const auto* yl = reinterpret_cast<const fpgemm16::h8*>(Yl);
+  hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<128, 128>), dim3((Nr / 128) * (Ncols / 128)),
+                     dim3(fpgemm16::NTHR), 0, S(stream), yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols,
                      alpha, rowscale, phase, bmax_in, cmax_out, reset_slot);
   return check_launch("hankel_gemm_mixed");
 }
@@ -722,8 +725,10 @@ int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb) 
   const fp_batch_instance& I = plan->host[i];
   const int m = bm ? bm : plan->bm[i], c = bn ? bn : plan->bn[i];
   const int t = nl_tpb ? nl_tpb : plan->nl_tpb[i];
-  if ((m != 64 && m != 128) || (c != 64 && c != 128) || I.Nr % m || (2 * I.Nt) % c)
-    return fail(FP_E_SHAPE, "fp_batch_set_config: tile must be 64 or 128 and divide Nr x 2 Nt");
+  const bool tile_ok = (m == 64 || m == 128) && (c == 64 || c == 128 || (c == 256 && m == 128));
+  if (!tile_ok || I.Nr % m || (2 * I.Nt) % c)
+    return fail(FP_E_SHAPE, "fp_batch_set_config: tile must be 64/128 x 64/128 or 128 x 256 and "
+                            "divide Nr x 2 Nt");
   if (t != I.Nt / 8 && t != I.Nt / 4)
     return fail(FP_E_ARG, "fp_batch_set_config: nonlinear threads must be Nt/8 or Nt/4");
   plan->bm[i] = m;

@@ -103,9 +103,9 @@ class MixedEngine:
         self.m = MixedTables(t64, cfg, g)
         dev = t64.device
         self.Nr, self.Nt = t64.Nr, t64.Nt
-        self.spec = torch.empty(self.Nr, self.Nt, dtype=torch.complex64, device=dev)
-        self.w0 = torch.empty_like(self.spec)
-        self.w1 = torch.empty_like(self.spec)
+        self.spec = torch.zeros(self.Nr, self.Nt, dtype=torch.complex64, device=dev)
+        self.w0 = torch.zeros_like(self.spec)
+        self.w1 = torch.zeros_like(self.spec)
         self.slots = torch.zeros(4, dtype=torch.int32, device=dev)
         self.set_time(u0_c128)
 
