@@ -14,7 +14,9 @@ with respect to the input field, by the discrete adjoint of every step:
 Gradients travel as FFT_t of the real-gradient field g (dL = Re sum conj(g) du),
 the same representation as the spectral field state. The forward states are
 checkpointed every ``ckpt`` steps (default ceil(sqrt(n))) and each segment is
-recomputed once during the backward sweep, so memory holds about 2 sqrt(n) states.
+recomputed once during the backward sweep, keeping per step the state, its carrier
+densities and the field after the first half linear step (what the step's adjoint
+linearises around), so memory holds about sqrt(n) states + 3 sqrt(n) step records.
 
 ``FluenceFromInput`` exposes the forward / backward pair as a torch.autograd.Function
 (input: complex64 time-domain field, output: F as float64). ``lg_modes`` /
@@ -99,32 +101,49 @@ class AdjointEngine:
             self._step()
         return self.fluence()
 
-    def _adjoint_step(self, un):
-        """gam <- gradient w.r.t. the state un (spectral) given gam w.r.t. step(un)."""
+    def _forward_keep(self, finish):
+        """One forward step from mx.spec, keeping what its adjoint needs: the state
+        (spectral), its carrier densities and v1 = the field after the first half
+        linear step (spectral). With finish=False the step stops after v1."""
         mx, it, st = self.mx, self.it, stream_ptr()
-        t = it.t
         mt = mx.m
-        mx.spec.copy_(un)
-        mx.slots.zero_()
-        call("fp_absmax_f32", ptr(mx.spec), 2 * self.Nr * self.Nt, mx._slot(0), st)
-        mx.plasma(it.rho, None)                                          # rho_n; clears slots 1-3
+        u = mx.spec.clone()
+        mx.plasma(it.rho, None)                                          # clears slots 1-3
         self._gemm(mt.Yh, mt.Yl, mt.y_scale_exp, mx.spec, mx.w1, mt.Rk, self.phase,
                    mx._slot(0), mx._slot(1), None)
         self._gemm(mt.Yh, mt.Yl, mt.y_scale_exp, mx.w1, mx.w0, mt.iRk, None,
-                   mx._slot(1), None, mx._slot(0))                       # w0 = v1
+                   mx._slot(1), None, mx._slot(0))
+        keep = (u, it.rho.clone(), mx.w0.clone())
+        if finish:
+            call("fp_nl_step_c64", ptr(mx.w0), ptr(mx.w0), ptr(it.rho), None, 1, 1,
+                 ctypes.byref(self.nlp), ctypes.byref(mt.tables), mx._slot(2), st)
+            self._gemm(mt.Yh, mt.Yl, mt.y_scale_exp, mx.w0, mx.w1, mt.Rk, self.phase,
+                       mx._slot(2), mx._slot(3), None)
+            call("fp_hankel_gemm_mixed", ptr(mt.Yh), ptr(mt.Yl), mt.y_scale_exp, ptr(mx.w1),
+                 ptr(mx.spec), self.Nr, self.Nt, ctypes.c_float(mt.iRk), ptr(mt.mask_r), None,
+                 mx._slot(3), mx._slot(0), None, st)
+        return keep
+
+    def _adjoint_step(self, un, rho_n, v1_n):
+        """gam <- gradient w.r.t. the state un (spectral) given gam w.r.t. step(un);
+        rho_n, v1_n: that step's carrier densities and field after the first half
+        linear step (spectral)."""
+        it, st = self.it, stream_ptr()
+        t = it.t
+        mt = self.mx.m
         self.aslots[1:].zero_()
         self._gemm(self.YMh, self.YMl, self.ym_se, self.gam, self.a1, mt.iRk, self.cphase,
                    self._aslot(0), self._aslot(1), None)
         self._gemm(self.YTh, self.YTl, self.yt_se, self.a1, self.g4, mt.Rk, None,
                    self._aslot(1), None, self._aslot(0))
-        call("fp_adj_nl_step_c64", ptr(mx.w0), ptr(it.rho), ptr(self.g4), ptr(self.g4),
+        call("fp_adj_nl_step_c64", ptr(v1_n), ptr(rho_n), ptr(self.g4), ptr(self.g4),
              ptr(self.grho), ptr(self.scratch), ctypes.byref(self.nlp), ctypes.byref(mt.tables),
              self._aslot(2), st)
         self._gemm(self.YTh, self.YTl, self.yt_se, self.g4, self.a1, mt.iRk, self.cphase,
                    self._aslot(2), self._aslot(3), None)
         self._gemm(self.YTh, self.YTl, self.yt_se, self.a1, self.gam, mt.Rk, None,
                    self._aslot(3), None, None)
-        call("fp_adj_plasma_c64", ptr(mx.spec), ptr(it.rho), ptr(self.grho), ptr(self.gam),
+        call("fp_adj_plasma_c64", ptr(un), ptr(rho_n), ptr(self.grho), ptr(self.gam),
              ctypes.byref(t.plasma_params), ptr(t.W_lut), ptr(mt.tw), self._aslot(0), st)
 
     def backward(self, dF):
@@ -138,21 +157,16 @@ class AdjointEngine:
         self.gam.copy_(c[:, None] * self.mx.spec)
         self.aslots.zero_()
         call("fp_absmax_f32", ptr(self.gam), 2 * self.Nr * self.Nt, self._aslot(0), stream_ptr())
-        starts = sorted(self._ckpts, reverse=True)
-        for s0 in starts:
+        for s0 in sorted(self._ckpts, reverse=True):
             s1 = min(s0 + K, n)
             self.mx.spec.copy_(self._ckpts[s0])
             self.mx.slots.zero_()
             call("fp_absmax_f32", ptr(self.mx.spec), 2 * self.Nr * self.Nt, self.mx._slot(0),
                  stream_ptr())
-            states = []
-            for i in range(s0, s1):
-                states.append(self.mx.spec.clone())
-                if i + 1 < s1:
-                    self._step()
+            kept = [self._forward_keep(finish=(i + 1 < s1)) for i in range(s0, s1)]
             for i in range(s1 - 1, s0 - 1, -1):
-                self._adjoint_step(states[i - s0])
-            del states
+                self._adjoint_step(*kept[i - s0])
+            del kept
         self._ckpts = None
         out = torch.empty_like(self.gam)
         call("fp_fft_rows_c64", ptr(self.gam), ptr(out), self.Nr, self.Nt, 1, ptr(self.mx.m.tw),

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -82,11 +83,19 @@ int dispatch_fft_rows(const T* in, T* out, int Nr, int Nt, int inverse, const T*
   }
 }
 
+// max |x_i| into *slot (float bits, atomicMax). Few blocks (one atomic each: the
+// atomics on one word serialise) and 16-byte loads where the length allows.
 __global__ void absmax_kernel(const float* __restrict__ x, long long n, unsigned* slot) {
   __shared__ float sred[4];
   float m = 0.0f;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long n4 = n / 4;
+  const float4* x4 = reinterpret_cast<const float4*>(x);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const float4 v = x4[i];
+    m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+  }
+  for (long long i = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     m = fmaxf(m, fabsf(x[i]));
   for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
   if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = m;
@@ -268,7 +277,10 @@ __global__ __launch_bounds__(fpgemm16::NTHR, (fpgemm16::TileOcc<BM, BN>::value))
   __shared__ fpgemm16::Smem<BM, BN> sm;
   const int total = g.begin[g.n];
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
-    const int item = (gridDim.x >= total) ? fpgemm16::xcd_remap(it, total) : it;
+    // XCD-aware order also on a persistent grid: with gridDim.x % 8 == 0, iteration `it`
+    // of any block runs on XCD it % 8, so the remap keeps each XCD on a contiguous range
+    // of tiles (whole column panels) and the panel's B rows stay in that XCD's L2.
+    const int item = (gridDim.x >= total || gridDim.x % 8 == 0) ? fpgemm16::xcd_remap(it, total) : it;
     const int j = group_find(g, item);
     const fp_batch_instance& I = insts[g.idx[j]];
     const int local = item - g.begin[j];
@@ -576,8 +588,9 @@ int fp_record_rows(const double* u, const double* rho, const double* rho_s, doub
 // ------------------------------------------------------------------ mixed precision
 int fp_absmax_f32(const float* x, long long n, unsigned* slot, void* stream) {
   if (!x || !slot || n < 0) return fail(FP_E_ARG, "fp_absmax_f32: bad argument");
-  long long blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
+  if (reinterpret_cast<uintptr_t>(x) % 16) return fail(FP_E_ARG, "fp_absmax_f32: x must be 16-byte aligned");
+  long long blocks = (n / 4 + 255) / 256;
+  if (blocks > 512) blocks = 512;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(256), 0, S(stream), x, n, slot);
   return check_launch("absmax");

@@ -126,7 +126,10 @@ def test_adjoint_step(pair):
     eng.gam.copy_(_c64(_spec(gn)))
     eng.aslots.zero_()
     call("fp_absmax_f32", ptr(eng.gam), 2 * eng.Nr * eng.Nt, eng._aslot(0), stream_ptr())
-    eng._adjoint_step(_c64(_spec(u)))
+    eng.mx.spec.copy_(_c64(_spec(u)))
+    eng.mx.slots.zero_()
+    call("fp_absmax_f32", ptr(eng.mx.spec), 2 * eng.Nr * eng.Nt, eng.mx._slot(0), stream_ptr())
+    eng._adjoint_step(*eng._forward_keep(finish=False))
     torch.cuda.synchronize()
     e = rel_l2(eng.gam.cpu().numpy(), expect)
     _announce(f"  rel-L2 {e:.2e}")
