@@ -65,6 +65,9 @@ typedef struct fp_nl_tables {
   const double* Rf;       /* Nt complex: Raman transfer function R_f          */
   const double* knots;    /* n_knots PCHIP abscissae (W/cm^2)                 */
   const double* pcoef;    /* 4 x (n_knots-1) PCHIP power-basis coefficients   */
+  const double* rw;       /* Nt+2 complex: w^m (m = 0..Nt), then (C, 0); the Raman
+                             response as a recursion, w = exp(dt (-1/tau_d + i/tau_s)),
+                             C = dt Omega_r2 tau_s (see raman.hpp)                 */
 } fp_nl_tables;
 
 /* Same tables in float32 for the mixed-precision path. PCHIP coefficients are
@@ -79,6 +82,7 @@ typedef struct fp_nl_tables_f32 {
   const float* knots;     /* n_knots */
   const float* inv_h;     /* n_knots-1: 1 / (x_{i+1} - x_i) */
   const float* pcoef_s;   /* 4 x (n_knots-1) */
+  const float* rw;        /* Nt+2 complex64: as fp_nl_tables.rw */
 } fp_nl_tables_f32;
 
 int fp_version(void);

@@ -36,13 +36,13 @@ class NLParams(ctypes.Structure):
 class NLTables(ctypes.Structure):
     """fp_nl_tables (float64 device pointers)."""
     _fields_ = [("tw", P), ("mA", P), ("mB", P), ("mC", P), ("Rf", P), ("knots", P),
-                ("pcoef", P)]
+                ("pcoef", P), ("rw", P)]
 
 
 class NLTablesF32(ctypes.Structure):
     """fp_nl_tables_f32."""
     _fields_ = [("tw", P), ("mA", P), ("mB", P), ("mC", P), ("Rf", P), ("knots", P),
-                ("inv_h", P), ("pcoef_s", P)]
+                ("inv_h", P), ("pcoef_s", P), ("rw", P)]
 
 
 class BatchInstance(ctypes.Structure):

@@ -124,14 +124,9 @@ FP_DEV void rhs_adj_row(const c32 (&y)[NT / TPB], const c32 (&lam)[NT / TPB], c3
     const float fR = (float)p.f_R, kp = (float)p.kerr_pref;
     float K[E];
     if (p.en_kerr_raman) {
+      fpraman::raman_row<float, NT, TPB>(y, K, sm.lds, reinterpret_cast<const c32*>(t.rw));
 #pragma unroll
-      for (int s = 0; s < E; ++s) buf[s] = mk<float>(y[s].x * y[s].x + y[s].y * y[s].y, 0.f);
-      row_fft<float, NT, TPB, false>(buf, sm.lds, tw);
-#pragma unroll
-      for (int s = 0; s < E; ++s) buf[s] = buf[s] * Rf[tid + s * TPB];
-      row_fft<float, NT, TPB, true>(buf, sm.lds, tw);
-#pragma unroll
-      for (int s = 0; s < E; ++s) K[s] = fR * buf[s].x * invN;
+      for (int s = 0; s < E; ++s) K[s] = fR * K[s];
     } else {
 #pragma unroll
       for (int s = 0; s < E; ++s) K[s] = 0.0f;

@@ -80,19 +80,28 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
   };
   Stage R0, R1;
 
+  // 32-bit byte offsets from the uniform base pointers (every operand here is below
+  // 4 GiB): loads take the scalar-base + 32-bit vector-offset form, no 64-bit address math.
+  const char* Bc = reinterpret_cast<const char*>(B);
+  const char* Yhc = reinterpret_cast<const char*>(Yh);
+  const char* Ylc = reinterpret_cast<const char*>(Yl);
+  const unsigned bcol4 = 4u * (unsigned)(n0 + bc);
+  const unsigned row4 = 4u * (unsigned)Ncols;
   auto gload = [&](int ks, Stage& R) {
-    const int k0 = ks * BK;
+    const unsigned k0 = (unsigned)(ks * BK);
 #pragma unroll
     for (int g = 0; g < GPT; ++g) {
-      const int kg = GPT * bkp + g;
+      const unsigned kg = (unsigned)(GPT * bkp + g);
+      const unsigned r0 = (k0 + 8u * kg) * row4 + bcol4;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) R.rb[g * 8 + j] = B[(size_t)(k0 + 8 * kg + j) * Ncols + n0 + bc];
+      for (int j = 0; j < 8; ++j)
+        R.rb[g * 8 + j] = *reinterpret_cast<const float*>(Bc + (r0 + (unsigned)j * row4));
     }
-    const size_t base = (size_t)ks * a_slab_stride + a_tile_off;
+    const unsigned base = 16u * ((unsigned)(ks * a_slab_stride + a_tile_off) + (unsigned)tid);
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
-      R.ra_h[i] = Yh[base + tid + i * NTHR];
-      R.ra_l[i] = Yl[base + tid + i * NTHR];
+      R.ra_h[i] = *reinterpret_cast<const h8*>(Yhc + (base + 16u * (unsigned)(i * NTHR)));
+      R.ra_l[i] = *reinterpret_cast<const h8*>(Ylc + (base + 16u * (unsigned)(i * NTHR)));
     }
   };
   auto sstore = [&](int buf, const Stage& R) {
@@ -170,7 +179,11 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
   }
 
   // epilogue: for v_mfma_f32_16x16x32_f16 lane l, register v holds D[4 (l/16) + v][l % 16]
+  // (32-bit byte offsets from the base pointers, as for the loads; the phase of the
+  // complex element (re, im) = columns (n & ~1, n | 1) is one 8-byte load)
   const float unscale = pow2f(-(y_scale_exp + sb_exp));
+  const char* Pc = reinterpret_cast<const char*>(phase);
+  char* Cc = reinterpret_cast<char*>(C);
   float vmax = 0.0f;
 #pragma unroll
   for (int a = 0; a < FA; ++a) {
@@ -178,18 +191,18 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
     for (int v = 0; v < 4; ++v) {
       const int m = m0 + wm + a * 16 + 4 * lq + v;
       const float rs = rowscale ? alpha * unscale * rowscale[m] : alpha * unscale;
+      const unsigned rowoff = 4u * (unsigned)m * (unsigned)Ncols;
 #pragma unroll
       for (int b = 0; b < FB; ++b) {
-        const int n = n0 + wn + b * 16 + lr;
+        const unsigned n = (unsigned)(n0 + wn + b * 16 + lr);
         float val = acc[a][b][v] * rs;
         if (phase) {
           const float other = __shfl_xor(val, 1, 64);
-          const size_t pidx = ((size_t)m * (Ncols >> 1) + (n >> 1)) * 2;
-          const float pr = phase[pidx], pi = phase[pidx + 1];
-          val = (n & 1) ? (other * pi + val * pr) : (val * pr - other * pi);
+          const float2 ph = *reinterpret_cast<const float2*>(Pc + rowoff + 4u * (n & ~1u));
+          val = (n & 1u) ? (other * ph.y + val * ph.x) : (val * ph.x - other * ph.y);
         }
         vmax = fmaxf(vmax, fabsf(val));
-        C[(size_t)m * Ncols + n] = val;
+        *reinterpret_cast<float*>(Cc + rowoff + 4u * n) = val;
       }
     }
   }

@@ -10,6 +10,7 @@
 #pragma once
 #include "common.hpp"
 #include "fft.hpp"
+#include "raman.hpp"
 #include "../../include/rapport_amd.h"
 
 namespace fpnl {
@@ -112,19 +113,14 @@ FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double* __restric
   }
   // Kerr intensity (instantaneous + Raman convolution), then A = i kappa kerr_I u
   if (p.en_kerr_inst || p.en_kerr_raman) {
-    if (p.en_kerr_raman) {
-#pragma unroll
-      for (int s = 0; s < E; ++s) buf[s] = mk<double>(us[s].x * us[s].x + us[s].y * us[s].y, 0.0);
-      row_fft<double, NT, TPB, false>(buf, lds, tw);
-#pragma unroll
-      for (int s = 0; s < E; ++s) buf[s] = buf[s] * Rf[tid + s * TPB];
-      row_fft<double, NT, TPB, true>(buf, lds, tw);
-    }
+    double KR[E];
+    if (p.en_kerr_raman)
+      fpraman::raman_row<double, NT, TPB>(us, KR, lds, reinterpret_cast<const cpx<double>*>(t.rw));
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
       double kerr = p.en_kerr_inst ? (1.0 - p.f_R) * a2 : 0.0;
-      if (p.en_kerr_raman) kerr = kerr + p.f_R * (buf[s].x * invN);
+      if (p.en_kerr_raman) kerr = kerr + p.f_R * KR[s];
       const double kk = p.kerr_pref * kerr;
       buf[s] = mk<double>(-kk * us[s].y, kk * us[s].x);
     }

@@ -9,6 +9,7 @@
 #pragma once
 #include "common.hpp"
 #include "fft.hpp"
+#include "raman.hpp"
 #include "../../include/rapport_amd.h"
 
 namespace fpnl32 {
@@ -104,20 +105,15 @@ FP_DEV void split_row(const c32 (&us)[NT / TPB], const float* rho_row,
     }
   }
   if (p.en_kerr_inst || p.en_kerr_raman) {
-    if (p.en_kerr_raman) {
-#pragma unroll
-      for (int s = 0; s < E; ++s) buf[s] = mk<float>(us[s].x * us[s].x + us[s].y * us[s].y, 0.f);
-      row_fft<float, NT, TPB, false>(buf, lds, tw);
-#pragma unroll
-      for (int s = 0; s < E; ++s) buf[s] = buf[s] * Rf[tid + s * TPB];
-      row_fft<float, NT, TPB, true>(buf, lds, tw);
-    }
+    float KR[E];
+    if (p.en_kerr_raman)
+      fpraman::raman_row<float, NT, TPB>(us, KR, lds, reinterpret_cast<const c32*>(t.rw));
     const float fR = (float)p.f_R, kp = (float)p.kerr_pref;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const float a2 = us[s].x * us[s].x + us[s].y * us[s].y;
       float kerr = p.en_kerr_inst ? (1.0f - fR) * a2 : 0.0f;
-      if (p.en_kerr_raman) kerr = kerr + fR * (buf[s].x * invN);
+      if (p.en_kerr_raman) kerr = kerr + fR * KR[s];
       const float kk = kp * kerr;
       buf[s] = mk<float>(-kk * us[s].y, kk * us[s].x);
     }

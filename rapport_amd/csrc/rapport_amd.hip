@@ -217,7 +217,7 @@ template <int NT> struct NLT32 { static constexpr int TPB = NT / 8; };
 // persistent grid (gridDim.x < items) each block loops over items.
 constexpr int kGroupMax = 128;
 // The descriptor is shared between host and device code: same layout on both sides.
-static_assert(sizeof(fp_batch_instance) == 432 && offsetof(fp_batch_instance, pp) == 136 &&
+static_assert(sizeof(fp_batch_instance) == 440 && offsetof(fp_batch_instance, pp) == 136 &&
                   offsetof(fp_batch_instance, np) == 240 && offsetof(fp_batch_instance, nt) == 368,
               "fp_batch_instance layout");
 struct BatchGroup {
@@ -696,7 +696,7 @@ int fp_batch_plan_create(const fp_batch_instance* inst, int n, fp_batch_plan** p
     }
     if (!I.Yh || !I.Yl || !I.spec || !I.w0 || !I.w1 || !I.rho || !I.phase || !I.mask_r ||
         !I.slots || !I.W_lut || !I.tw || !I.nt.tw || !I.nt.mA || !I.nt.mB || !I.nt.mC ||
-        !I.nt.Rf || !I.nt.knots || !I.nt.inv_h || !I.nt.pcoef_s ||
+        !I.nt.Rf || !I.nt.knots || !I.nt.inv_h || !I.nt.pcoef_s || !I.nt.rw ||
         (I.enable_ste && (!I.rho_s || !I.Ws_lut))) {
       snprintf(msg, sizeof msg, "fp_batch_plan_create: instance %d: null pointer", i);
       return fail(FP_E_ARG, msg);

@@ -61,6 +61,7 @@ class MixedTables:
         T, iU = g["T_op"], g["inv_U_nl"]
         self.mA, self.mB, self.mC = f32(T * T * iU), f32(-T * iU), f32(iU)
         self.Rf = c64(g["R_f"])
+        self.rw = c64(t64.rw_host)
         sp = g["keldysh"]["f_spline"]
         x = sp.x
         h = np.diff(x)
@@ -78,7 +79,7 @@ class MixedTables:
         self._phase = {}
         self.tables = _lib.NLTablesF32(ptr(self.tw), ptr(self.mA), ptr(self.mB), ptr(self.mC),
                                        ptr(self.Rf), ptr(self.knots), ptr(self.inv_h),
-                                       ptr(self.pcoef_s))
+                                       ptr(self.pcoef_s), ptr(self.rw))
 
     def phase_half(self, dz):
         """exp(i phi dz/2) * spec_mask with phi evaluated in FP64 and reduced mod 2 pi

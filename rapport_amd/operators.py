@@ -38,6 +38,19 @@ def _c128(a, device):
     return torch.from_numpy(a).to(device)
 
 
+def raman_recursion_table(cfg, g):
+    """Nt + 2 complex entries: w^m for m = 0..Nt, then (C, 0), where the sampled Raman
+    response of grids.py is h[k] = C Im(w^k) (k = 1..Nt/2-1), w = exp(dt (-1/tau_d +
+    i/tau_s)), C = dt Omega_r2 tau_s: the kernels evaluate the Raman convolution
+    Re(IFFT(FFT(I) R_f)) as the equivalent first-order recursion (csrc/raman.hpp)."""
+    Nt, dt = cfg.Nt, g["dt"]
+    m = np.arange(Nt + 1)
+    out = np.empty(Nt + 2, dtype=np.complex128)
+    out[:Nt + 1] = np.exp(m * dt * (-1.0 / cfg.tau_d + 1j / cfg.tau_s))
+    out[Nt + 1] = dt * cfg.Omega_r2 * cfg.tau_s
+    return out
+
+
 class DeviceTables:
     """Per-run device tables and kernel parameter blocks."""
 
@@ -58,6 +71,8 @@ class DeviceTables:
         self.mB = _dev(-T * iU, torch.float64, dev)
         self.mC = _dev(iU, torch.float64, dev)
         self.Rf = _c128(g["R_f"], dev)
+        self.rw_host = raman_recursion_table(cfg, g)
+        self.rw = _c128(self.rw_host, dev)
         kel = g["keldysh"]
         sp = kel["f_spline"]
         self.n_knots = len(sp.x)
@@ -84,7 +99,7 @@ class DeviceTables:
         self.plasma_params = pp
 
         self.tables = _lib.NLTables(ptr(self.tw), ptr(self.mA), ptr(self.mB), ptr(self.mC),
-                                    ptr(self.Rf), ptr(self.knots), ptr(self.pcoef))
+                                    ptr(self.Rf), ptr(self.knots), ptr(self.pcoef), ptr(self.rw))
 
     def nl_params(self, dz, absorb=None):
         cfg, g = self.cfg, self.g

@@ -35,7 +35,7 @@ def _desc(**over):
     d.np.Nr, d.np.Nt = 256, 1024
     for f in ("Yh", "Yl", "spec", "w0", "w1", "rho", "phase", "mask_r", "slots", "W_lut", "tw"):
         setattr(d, f, 0x1000)
-    for f in ("tw", "mA", "mB", "mC", "Rf", "knots", "inv_h", "pcoef_s"):
+    for f in ("tw", "mA", "mB", "mC", "Rf", "knots", "inv_h", "pcoef_s", "rw"):
         setattr(d.nt, f, 0x1000)
     for k, v in over.items():
         setattr(d, k, v)
