@@ -271,6 +271,9 @@ typedef struct fp_batch_instance {
   fp_plasma_params pp;
   fp_nl_params np;
   fp_nl_tables_f32 nt;
+  int zc_lo, zc_hi;           /* real columns [zc_lo, zc_hi) of every GEMM output are exactly
+                                 zero (frequencies where the spectral mask is 0): those
+                                 tiles are written as zeros without the product; 0, 0 = none */
 } fp_batch_instance;
 
 typedef struct fp_batch_plan fp_batch_plan;   /* opaque */

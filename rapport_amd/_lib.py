@@ -51,7 +51,8 @@ class BatchInstance(ctypes.Structure):
                 ("Rk", ctypes.c_float), ("iRk", ctypes.c_float), ("Yh", P), ("Yl", P),
                 ("spec", P), ("w0", P), ("w1", P), ("rho", P), ("rho_s", P), ("phase", P),
                 ("mask_r", P), ("slots", P), ("W_lut", P), ("Ws_lut", P), ("tw", P),
-                ("status", P), ("pp", PlasmaParams), ("np", NLParams), ("nt", NLTablesF32)]
+                ("status", P), ("pp", PlasmaParams), ("np", NLParams), ("nt", NLTablesF32),
+                ("zc_lo", c_int), ("zc_hi", c_int)]
 
 
 # name -> argtypes (restype is int for all but fp_last_error)

@@ -36,6 +36,18 @@ def _vp(t):
     return None if t is None else t.data_ptr()
 
 
+def zero_columns(spec_mask):
+    """Real-column range [lo, hi) of the frequencies where the spectral mask is exactly 0
+    (the longest contiguous run; (0, 0) if none): every Hankel GEMM output of the step is
+    exactly zero there (the half-step phase carries the mask)."""
+    z = np.flatnonzero(np.asarray(spec_mask) == 0.0)
+    if z.size == 0:
+        return 0, 0
+    runs = np.split(z, np.flatnonzero(np.diff(z) != 1) + 1)
+    r = max(runs, key=len)
+    return 2 * int(r[0]), 2 * (int(r[-1]) + 1)
+
+
 class BatchedIntegrator:
     """``members``: list of mixed-precision Integrators advanced in lock-step.
 
@@ -94,6 +106,7 @@ class BatchedIntegrator:
         d.pp = t.plasma_params
         d.np = nlp
         d.nt = mt.tables
+        d.zc_lo, d.zc_hi = zero_columns(m.g["spec_mask"])
 
     def set_config(self, i, bm=0, bn=0, nl_tpb=0):
         """Launch configuration of instance i (GEMM tile bm x bn, nonlinear threads)."""

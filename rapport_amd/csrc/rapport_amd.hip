@@ -217,7 +217,7 @@ template <int NT> struct NLT32 { static constexpr int TPB = NT / 8; };
 // persistent grid (gridDim.x < items) each block loops over items.
 constexpr int kGroupMax = 128;
 // The descriptor is shared between host and device code: same layout on both sides.
-static_assert(sizeof(fp_batch_instance) == 440 && offsetof(fp_batch_instance, pp) == 136 &&
+static_assert(sizeof(fp_batch_instance) == 448 && offsetof(fp_batch_instance, pp) == 136 &&
                   offsetof(fp_batch_instance, np) == 240 && offsetof(fp_batch_instance, nt) == 368,
               "fp_batch_instance layout");
 struct BatchGroup {
@@ -294,8 +294,20 @@ __global__ __launch_bounds__(fpgemm16::NTHR, (fpgemm16::TileOcc<BM, BN>::value))
     const unsigned* bin = I.slots + STAGE;
     unsigned* cout = STAGE == 0 ? I.slots + 1 : STAGE == 2 ? I.slots + 3 : STAGE == 3 ? I.slots : nullptr;
     if (STAGE == 1 && local == 0 && threadIdx.x == 0) I.slots[0] = 0u;
-    fpgemm16::gemm_tile<BM, BN>(sm, local % tilesM, local / tilesM,
-                                reinterpret_cast<const fpgemm16::h8*>(I.Yh),
+    const int tm = local % tilesM, tn = local / tilesM;
+    if (tn * BN >= I.zc_lo && (tn + 1) * BN <= I.zc_hi) {
+      // masked frequencies: the product is exactly zero there (phase x mask = 0 in G1 / G3,
+      // zero input columns in G2 / G4); write the zeros, skip the MFMA work
+      float4* Cz = reinterpret_cast<float4*>(C);
+      const int ld4 = (2 * I.Nt) / 4;
+      for (int i = threadIdx.x; i < BM * BN / 4; i += fpgemm16::NTHR) {
+        const int r = i / (BN / 4), c4 = i % (BN / 4);
+        Cz[(size_t)(tm * BM + r) * ld4 + (tn * BN) / 4 + c4] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      __syncthreads();
+      continue;
+    }
+    fpgemm16::gemm_tile<BM, BN>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(I.Yh),
                                 reinterpret_cast<const fpgemm16::h8*>(I.Yl), I.y_scale_exp, B, C,
                                 I.Nr, I.Nr, 2 * I.Nt, alpha, rs, ph, bin, cout);
     __syncthreads();
@@ -689,7 +701,8 @@ int fp_batch_plan_create(const fp_batch_instance* inst, int n, fp_batch_plan** p
       return fail(FP_E_SHAPE, msg);
     }
     if (I.pp.Nr != I.Nr || I.pp.Nt != I.Nt || I.np.Nr != I.Nr || I.np.Nt != I.Nt ||
-        I.pp.enable_ste != I.enable_ste || I.pp.table_size < 2) {
+        I.pp.enable_ste != I.enable_ste || I.pp.table_size < 2 || I.zc_lo < 0 ||
+        I.zc_hi < I.zc_lo || I.zc_hi > 2 * I.Nt) {
       snprintf(msg, sizeof msg, "fp_batch_plan_create: instance %d: parameter blocks disagree with "
                "its shape / STE flag", i);
       return fail(FP_E_ARG, msg);
