@@ -21,8 +21,9 @@
 // guesses are the row's previous densities (the previous z-step's, when the
 // row buffer holds them), which makes the first iteration nearly exact. The
 // converged starts agree to 1e-14 relative whatever the guess, so results depend
-// on the previous buffer contents only at that level. A final pass from the
-// converged starts writes n_e, n_s.
+// on the previous buffer contents only at that level. Each iteration writes its
+// values to the LDS rows; the last iteration's (from starts within 1e-14 of the
+// converged ones) are the result.
 #pragma once
 #include "common.hpp"
 #include "../../include/rapport_amd.h"
@@ -197,6 +198,8 @@ FP_DEV int plasma_row(double* smem, const fp_plasma_params& p, const double* __r
     }
   }
 
+  __syncthreads();   // every thread has read its intensities before the rows are overwritten
+
   // First guess of this chunk's start value: the row's previous content (the
   // densities of the previous z-step, which the field changes little), else 0.
   St start;
@@ -215,12 +218,19 @@ FP_DEV int plasma_row(double* smem, const fp_plasma_params& p, const double* __r
   for (;;) {
     ++it;
     // integrate the chunk with tangent
+    // (the values go to the LDS rows as they are produced: once the starts have
+    // converged, the last iteration's values are the row -- no separate final pass)
     Map J;
     J.a00 = 1.0; J.a01 = 0.0; J.a10 = 0.0; J.a11 = 1.0; J.b0 = J.b1 = 0.0;
     St x = start;
 #pragma unroll
-    for (int i = 0; i < C; ++i)
-      if (i < nsteps) x = rate_step<STE, true>(x, Ia[i], Wa[i], Wsa[i], p, J);
+    for (int i = 0; i < C; ++i) {
+      if (i < nsteps) {
+        x = rate_step<STE, true>(x, Ia[i], Wa[i], Wsa[i], p, J);
+        sI[k0 + i + 1] = x.e;
+        if (STE) sS[k0 + i + 1] = x.s;
+      }
+    }
     // affine map of chunk: s_{c+1} = J (s_c - start) + x
     Map m = J;
     m.b0 = x.e - (J.a00 * start.e + (STE ? J.a01 * start.s : 0.0));
@@ -277,20 +287,7 @@ FP_DEV int plasma_row(double* smem, const fp_plasma_params& p, const double* __r
     }
   }
 
-  // final exact pass from the converged starts -> LDS rows
-  {
-    Map J;
-    St x = start;
-    if (k0 == 0) { sI[0] = 0.0; if (STE) sS[0] = 0.0; }
-#pragma unroll
-    for (int i = 0; i < C; ++i) {
-      if (i < nsteps) {
-        x = rate_step<STE, false>(x, Ia[i], Wa[i], Wsa[i], p, J);
-        sI[k0 + i + 1] = x.e;
-        if (STE) sS[k0 + i + 1] = x.s;
-      }
-    }
-  }
+  if (k0 == 0) { sI[0] = 0.0; if (STE) sS[0] = 0.0; }
   __syncthreads();
   for (int k = tid; k < NT; k += TPB) {
     ne_row[k] = sI[k];
