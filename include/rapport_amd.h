@@ -304,6 +304,52 @@ int fp_batch_step_mixed(fp_batch_plan* plan, void* stream);
 /* Number of kernel launches one fp_batch_step_mixed makes (after grouping). */
 int fp_batch_launches(const fp_batch_plan* plan);
 
+/* ================================================================ 0-D rate re-integration
+ * Batched replacement of integre_0d (D3/sim/figures_article.py:168-194, with _phi1 / _step
+ * :161-166): the electron and STE densities re-integrated over given intensity histories
+ * I(t), one independent row per history (fig13_electron_density_vs_z calls it once per
+ * saved z plane and model variant, :297-303). Per step k -> k+1: averages of I, W, W_s
+ * over the two samples, the exact exponential step n <- e^{L dt} n + S dt phi1(L dt)
+ * (0 when not finite or negative), then n_e, n_s rescaled when n_e + n_s > rho_max. */
+typedef struct fp_rate0d_params {
+  int n_rows, Nt;                 /* intensity histories x samples per history (Nt >= 1) */
+  double na;                      /* rho_max [cm^-3] */
+  double inv_tau_r;               /* 1 / tau_r [1/s] */
+  double beta_g, beta_s;          /* sigma / Ui, sigma / Us [cm^2/J] */
+  int avalanche, trapping, ste;   /* model variants (avalanche, piegeage, ste) */
+  int n_knots;                    /* PCHIP knots in log10 I (>= 2) */
+  double Imin, Imax;              /* clip range of I for the rates [W/cm^2] */
+} fp_rate0d_params;
+
+/* t: Nt sample times [s], shared by every row; I: n_rows x Nt intensities [W/cm^2];
+ * knots: n_knots increasing log10 I knots; coef_g, coef_s: PCHIP coefficients of the
+ * valence-band and STE rates, 4 x (n_knots - 1), row j multiplying (x - knot_i)^(3 - j)
+ * (scipy PPoly order); coef_s is read only when ste. Rate = max(pchip(log10(clip(I))), 0).
+ * ne, ns: n_rows x Nt outputs (ns holds zeros when ste == 0). All device pointers. */
+int fp_rate0d_batch(const double* t, const double* I, double* ne, double* ns,
+                    const fp_rate0d_params* p, const double* knots, const double* coef_g,
+                    const double* coef_s, void* stream);
+
+/* ================================================================ 1-D SPM + shock model
+ * Batched replacement of propagate_shock (D3/sim/self_steepening.py:80-105, right-hand
+ * side shock_rhs :65-77 and _dt_spectral :60-62): classical RK4 in z of
+ *   dE/dz = i spm |E|^2 E - shock ifft(i w fft(|E|^2 E)),  w = 2 pi fftfreq(N, dt),
+ * one workgroup per pulse for the whole z march (field in registers, transforms in LDS). */
+typedef struct fp_shock_row {
+  double dt;      /* sample spacing [s] */
+  double dz;      /* z step [m] */
+  double spm;     /* K0 n2 (E in sqrt(W/m^2)) */
+  double shock;   /* n2 / c; 0 = SPM only */
+} fp_shock_row;
+
+/* E0: n_rows x N complex128 input pulses; rows: n_rows parameter blocks; out: n_rows x
+ * n_saves x N complex128 with plane 0 = E0 and plane j the field after j * stride steps,
+ * n_saves = 1 + nz / stride; work: n_rows x 3N complex128 scratch, used for N = 8192 (may be
+ * NULL otherwise); tw: N complex128 twiddles exp(-2 pi i m / N). N in {1024, 2048, 4096,
+ * 8192}. All device pointers. */
+int fp_shock_propagate(const double* E0, double* out, double* work, const fp_shock_row* rows,
+                       int n_rows, int N, int nz, int stride, const double* tw, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

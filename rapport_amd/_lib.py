@@ -55,6 +55,14 @@ class BatchInstance(ctypes.Structure):
                 ("zc_lo", c_int), ("zc_hi", c_int)]
 
 
+class Rate0DParams(ctypes.Structure):
+    """fp_rate0d_params (0-D re-integration over intensity histories)."""
+    _fields_ = [("n_rows", c_int), ("Nt", c_int), ("na", c_double), ("inv_tau_r", c_double),
+                ("beta_g", c_double), ("beta_s", c_double), ("avalanche", c_int),
+                ("trapping", c_int), ("ste", c_int), ("n_knots", c_int), ("Imin", c_double),
+                ("Imax", c_double)]
+
+
 # name -> argtypes (restype is int for all but fp_last_error)
 _SIGS = {
     "fp_version": [],
@@ -84,6 +92,8 @@ _SIGS = {
     "fp_batch_pass": [P, c_int, P],
     "fp_batch_step_mixed": [P, P],
     "fp_batch_launches": [P],
+    "fp_rate0d_batch": [P, P, P, P, ctypes.POINTER(Rate0DParams), P, P, P, P],
+    "fp_shock_propagate": [P, P, P, P, c_int, c_int, c_int, c_int, P, P],
 }
 
 _lib = None

@@ -19,6 +19,8 @@
 #include "nl.hpp"
 #include "nl_f32.hpp"
 #include "plasma.hpp"
+#include "rate0d.hpp"
+#include "shock1d.hpp"
 
 #define FP_VERSION 100
 
@@ -834,6 +836,60 @@ int fp_adj_plasma_c64(const float* us, const double* rho, const float* grho, flo
     default: return fail(FP_E_SHAPE, "fp_adj_plasma_c64: Nt must be 512, 1024, 2048 or 4096");
   }
 #undef FP_ADJPL
+}
+
+// ------------------------------------------------------------------ 0-D re-integration
+int fp_rate0d_batch(const double* t, const double* I, double* ne, double* ns,
+                    const fp_rate0d_params* p, const double* knots, const double* coef_g,
+                    const double* coef_s, void* stream) {
+  if (!p) return fail(FP_E_ARG, "fp_rate0d_batch: null params");
+  if (p->n_rows < 0 || p->Nt < 1 || p->n_knots < 2)
+    return fail(FP_E_SHAPE, "fp_rate0d_batch: needs n_rows >= 0, Nt >= 1, n_knots >= 2");
+  if (!(p->na > 0.0) || !(p->Imin > 0.0) || !(p->Imax >= p->Imin))
+    return fail(FP_E_ARG, "fp_rate0d_batch: needs na > 0 and 0 < Imin <= Imax");
+  if (p->n_rows == 0) return FP_OK;   // empty batch: nothing is read or written
+  if (!t || !I || !ne || !ns || !knots || !coef_g)
+    return fail(FP_E_ARG, "fp_rate0d_batch: null pointer");
+  if (p->ste && !coef_s) return fail(FP_E_ARG, "fp_rate0d_batch: STE needs coef_s");
+  const unsigned blocks = (unsigned)((p->n_rows + fprate0d::ROWS - 1) / fprate0d::ROWS);
+  hipLaunchKernelGGL(fprate0d::rate0d_kernel, dim3(blocks), dim3(fprate0d::ROWS), 0, S(stream), t,
+                     I, ne, ns, *p, knots, coef_g, p->ste ? coef_s : coef_g);
+  return check_launch("rate0d_batch");
+}
+
+// ------------------------------------------------------------------ 1-D SPM + shock
+int fp_shock_propagate(const double* E0, double* out, double* work, const fp_shock_row* rows,
+                       int n_rows, int N, int nz, int stride, const double* tw, void* stream) {
+  if (n_rows < 0 || nz < 0 || stride < 1)
+    return fail(FP_E_ARG, "fp_shock_propagate: needs n_rows >= 0, nz >= 0, stride >= 1");
+  if (N != 1024 && N != 2048 && N != 4096 && N != 8192)
+    return fail(FP_E_SHAPE, "fp_shock_propagate: N must be 1024, 2048, 4096 or 8192");
+  if (n_rows == 0) return FP_OK;   // empty batch: nothing is read or written
+  if (!E0 || !out || !rows || !tw) return fail(FP_E_ARG, "fp_shock_propagate: null pointer");
+  if (N == 8192 && !work) return fail(FP_E_ARG, "fp_shock_propagate: N = 8192 needs work");
+  const int n_saves = 1 + nz / stride;
+  hipStream_t st = S(stream);
+  switch (N) {
+    case 1024:
+      hipLaunchKernelGGL((fpshock::shock_kernel<1024, 128, false>), dim3(n_rows), dim3(128), 0, st,
+                         E0, out, work, rows, nz, stride, n_saves, tw);
+      break;
+    case 2048:
+      hipLaunchKernelGGL((fpshock::shock_kernel<2048, 256, false>), dim3(n_rows), dim3(256), 0, st,
+                         E0, out, work, rows, nz, stride, n_saves, tw);
+      break;
+    case 4096:
+      hipLaunchKernelGGL((fpshock::shock_kernel<4096, 512, false>), dim3(n_rows), dim3(512), 0, st,
+                         E0, out, work, rows, nz, stride, n_saves, tw);
+      break;
+    case 8192:
+      hipLaunchKernelGGL((fpshock::shock_kernel<8192, 1024, true>), dim3(n_rows), dim3(1024), 0, st,
+                         E0, out, work, rows, nz, stride, n_saves, tw);
+      break;
+    default:
+      return fail(FP_E_SHAPE, "fp_shock_propagate: N must be 1024, 2048, 4096 or 8192");
+  }
+  return check_launch("shock_propagate");
 }
 
 }  // extern "C"
