@@ -139,6 +139,23 @@ def test_split_vs_oracle(ste):
     assert rel_l2(pl.cpu().numpy(), pl_o) < 1e-14
 
 
+def test_split_raman_only_vs_oracle():
+    """Raman term alone: the device's recursion scan against the oracle's
+    ifft(fft(I) R_f) form of the reference."""
+    _announce("split C1 raman only")
+    only = dict(enable_kerr_instantaneous=False, enable_kerr_raman=True,
+                enable_photoionization_loss=False, enable_plasma_defocusing=False,
+                enable_plasma_absorption=False, enable_ste=False)
+    R, cfg_p, cfg_o, g_p, g_o, tabs = _setup(C1, **only)
+    Nr, Nt = g_p["Y"].shape[0], cfg_p.Nt
+    u = pulse_field(Nr, Nt, g_p["dt"], g_p["invE2"], np.linspace(1e12, 7e13, Nr), seed=4)
+    rho = np.zeros((Nr, Nt))
+    rhs_o, _ = O.NonlinearOperator(cfg_o, g_o).split(u, rho)
+    rhs, _ = R.NonlinearOperator(cfg_p, g_p, tabs).split(_cuda(u), _cuda(rho))
+    assert np.abs(rhs_o).max() > 0
+    assert rel_l2(rhs.cpu().numpy(), rhs_o) < 1e-11
+
+
 def test_split_toggles_off_is_zero():
     _announce("split toggles off")
     off = dict(enable_kerr_instantaneous=False, enable_kerr_raman=False, enable_self_steepening=False,
