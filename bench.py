@@ -299,14 +299,34 @@ def copy_calibration(nbytes=256 * 2 ** 20, reps=5):
     print(json.dumps(dict(copy_bytes=nbytes, reps=reps)), flush=True)
 
 
+def _pmc_traffic(kernel_substr, section="zstep_1024x4096"):
+    """HBM bytes per launch of a kernel from the committed PMC summary (rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        data = json.load(open(path))[section]
+    except (OSError, KeyError, ValueError):
+        return None
+    for name, v in data.items():
+        if kernel_substr in name:
+            return dict(bytes=v["read_bytes"] + v["write_bytes"], read_bytes=v["read_bytes"],
+                        write_bytes=v["write_bytes"], kernel=name,
+                        source="profiles/pmc_traffic.json")
+    return None
+
+
 def _roofline(prec, r):
     Nr, Nt = r["Nr"], r["Nt"]
     flops = 4.0 * Nr * Nr * Nt                          # real Y (Nr x Nr) @ complex (Nr x Nt)
     ach = flops / (r["gemm_ms"] * 1e-3) / 1e12
     peak = PEAK_TFLOPS[prec]
+    kname = "hankel_gemm_f64" if prec == "fp64" else "hankel_gemm_f16x3"
+    tr = _pmc_traffic(kname)
     roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                traffic=None, kernel="hankel_gemm_f64" if prec == "fp64" else "hankel_gemm_f16x3",
+                traffic=tr["bytes"] if tr else None, kernel=kname,
                 flops_per_launch=flops, avg_launch_ms=r["gemm_ms"], launches_timed=r["n_gemm"])
+    if tr:
+        roof["traffic_detail"] = tr
     if prec == "mixed":
         roof["executed_flops_per_launch"] = 3 * flops
         roof["executed_frac"] = 3 * ach / peak
