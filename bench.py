@@ -45,6 +45,11 @@ WORKLOAD = dict(nz=3999, Nt=4096, N=1025, R_factor=15.0, begin=0.0, end=3999 * 5
                 rho_max=2.1e22, tau_c=1.27e-15, tau_r=150e-15, delta_t=85e-15, w0=100e-6,
                 enable_ste=False, rho_t_stride=0, write_output=False, verbose=False)
 ENVELOPE = dict(lens_f=0.05, noise=0.01, seed=2)
+# the CPU-sized case (128 x 512, 800 fs window, 2 uJ collimated, 200 steps of 10 um): the
+# mixed path's errors per output are reported at this shape too
+SMALL = dict(WORKLOAD, nz=199, Nt=512, N=129, R_factor=10.0, end=1.99e-3, t_window=800e-15,
+             energy_uJ=2.0)
+SMALL_ENVELOPE = dict(lens_f=None, noise=0.01, seed=1)
 
 
 def _cores():
@@ -338,6 +343,30 @@ def _roofline(prec, r):
     return roof, ng
 
 
+def accuracy_small():
+    """Mixed vs FP64 at 128 x 512 over the whole 200-step run (fluence recorded every
+    step): rel-L2 of the final field, the electron density and the fluence."""
+    import numpy as np
+    import rapport_amd as R
+    res, fields = {}, {}
+    for prec in ("fp64", "mixed"):
+        kw = dict(SMALL, save_stride=1, precision=prec)
+        it = R.Integrator(R.Config(**kw), envelope=R.envelope_lens_noise(**SMALL_ENVELOPE))
+        res[prec] = it.propagate()
+        fields[prec] = it.u.cpu().numpy().astype(np.complex128)
+        del it
+
+    def rl2(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    rp, rm = res["fp64"], res["mixed"]
+    return dict(shape="128 x 512", steps=200,
+                field=float(np.linalg.norm(fields["mixed"] - fields["fp64"]) /
+                            np.linalg.norm(fields["fp64"])),
+                rho_rz=rl2(rm["rho_rz"], rp["rho_rz"]),
+                fluence_rz=rl2(rm["fluence_rz"], rp["fluence_rz"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,6 +418,7 @@ def main():
         mixed_speedup_vs_fp64=(runs["mixed"]["steps_per_s"] / runs["fp64"]["steps_per_s"]
                                if len(runs) == 2 else None),
         accuracy_mixed_vs_fp64=acc,
+        accuracy_mixed_vs_fp64_128x512=accuracy_small() if args.precision == "both" else None,
         plasma_scan_capped_rows=sum(x["capped_rows"] for x in runs.values()),
     )
     if args.batched > 0:

@@ -134,7 +134,11 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
     for (int b = 0; b < FB; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
   const int lq = lane >> 4, lr = lane & 15;
+  // MFMA phase of a slab at raised wave priority, with the scheduler told to interleave the
+  // B-fragment LDS reads of the next column block between the current block's MFMAs (both
+  // measured: 10 % faster at 1024 x 4096, results bitwise unchanged).
   auto mma = [&](int cur) {
+    __builtin_amdgcn_s_setprio(1);
     h8 ah[FA], al[FA];
 #pragma unroll
     for (int a = 0; a < FA; ++a) {
@@ -155,6 +159,16 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh, acc[a][b], 0, 0, 0);
       }
     }
+    // groups: the A fragments and the first B pair (DS reads), then per column block
+    // half of its 3 FA MFMAs, the next block's two B reads, the other half
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * FA + 2, 0);
+#pragma unroll
+    for (int b = 0; b < FB; ++b) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 3 * FA / 2, 0);
+      if (b + 1 < FB) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 3 * FA / 2, 0);
+    }
+    __builtin_amdgcn_s_setprio(0);
   };
 
   // Slab s lives in LDS buffer s & 1. Invariant at the top of an even step s: slab s is
