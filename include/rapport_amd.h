@@ -81,7 +81,7 @@ typedef struct fp_nl_tables_f32 {
   const float* Rf;        /* Nt complex64 */
   const float* knots;     /* n_knots */
   const float* inv_h;     /* n_knots-1: 1 / (x_{i+1} - x_i) */
-  const float* pcoef_s;   /* 4 x (n_knots-1) */
+  const float* pcoef_s;   /* (n_knots-1) x 4 interleaved (c0..c3 per interval), 16-byte aligned */
   const float* rw;        /* Nt+2 complex64: as fp_nl_tables.rw */
 } fp_nl_tables_f32;
 

@@ -42,9 +42,8 @@ FP_DEV float photo_rate_d(float a2, float rho, const fp_nl_params& p, const fp_n
   const int n = p.n_knots;
   float s, ipk;
   const int i = fpnl32::pchip_piece(x, n, s, ipk);
-  const int m = n - 1;
-  const float c0 = t.pcoef_s[i], c1 = t.pcoef_s[m + i], c2 = t.pcoef_s[2 * m + i],
-              c3 = t.pcoef_s[3 * m + i];
+  const float4 c = fpnl32::pchip_coef(t, i);
+  const float c0 = c.x, c1 = c.y, c2 = c.z, c3 = c.w;
   const float f = ((c0 * s + c1) * s + c2) * s + c3;
   float dw = ((3.0f * c0 * s + 2.0f * c1) * s + c2) * (100.0f * ipk);
   if (f < 0.0f) dw = -dw;

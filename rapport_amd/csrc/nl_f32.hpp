@@ -45,6 +45,12 @@ FP_DEV int pchip_piece(float x, int n, float& s, float& inv_pk) {
   return i;
 }
 
+// Coefficients (c0, c1, c2, c3) of PCHIP interval i: one 16-byte load from the
+// interleaved (n_knots - 1) x 4 table.
+FP_DEV float4 pchip_coef(const fp_nl_tables_f32& t, int i) {
+  return reinterpret_cast<const float4*>(t.pcoef_s)[i];
+}
+
 // |W(I)| * photo_pref * depl / (|u|^2 + 1e-30), the photo-ionisation loss rate.
 // The knot grid is 10^k (1 + 0.01 j), so the interval and the local coordinate
 // follow from the decade alone: v = (I / 10^k - 1) * 100, j = floor(v),
@@ -55,10 +61,8 @@ FP_DEV float photo_rate(float a2, float rho, const fp_nl_params& p, const fp_nl_
   const int n = p.n_knots;
   float sl, ipk;
   const int i = pchip_piece(x, n, sl, ipk);
-  const int m = n - 1;
-  const float c0 = t.pcoef_s[i], c1 = t.pcoef_s[m + i], c2 = t.pcoef_s[2 * m + i],
-              c3 = t.pcoef_s[3 * m + i];
-  const float w = fabsf(((c0 * sl + c1) * sl + c2) * sl + c3);
+  const float4 c = pchip_coef(t, i);
+  const float w = fabsf(((c.x * sl + c.y) * sl + c.z) * sl + c.w);
   float depl = 1.0f - rho * rcp((float)p.rho_max);
   depl = fminf(fmaxf(depl, 0.0f), 1.0f);
   return nan0(w) * depl * rcp(a2 + 1e-30f);

@@ -13,7 +13,8 @@
 // The recursion runs as a workgroup scan: each thread folds a contiguous chunk of E
 // samples into one affine map P -> w^E P + B (the multiplier is the same for every
 // chunk), a weighted inclusive scan combines the chunks (wave shuffles, then wave
-// totals through LDS), the periodic start is P[0] = X_total / (1 - w^N), and each
+// totals through LDS, carried across waves by the same recursion), the periodic start is
+// P[0] = X_total / (1 - w^N), and each
 // thread regenerates its chunk. Input and output use the FFT stride layout
 // (v[s] = x[tid + s TPB]); sample n - N/2 is then register (s + E/2) mod E of the
 // same thread. Replaces a forward and an inverse row FFT.
@@ -78,7 +79,15 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   __syncthreads();                            // chunk reads of lf done
   if (lane == 63) lds[wave] = X;
   __syncthreads();
-  for (int v = 0; v < wave; ++v) X = X + wt[E * (tid - 64 * v - 63)] * lds[v];
+  // carry from the earlier waves: G = scan value at the last thread of wave - 1, built
+  // from the wave totals T_v by G_v = T_v + w^(64 E) G_{v-1} (broadcast LDS reads), then
+  // X += w^(E (lane + 1)) G -- one per-lane table read instead of one per earlier wave
+  if (wave > 0) {
+    const cpx<T> w64 = wt[64 * E];
+    cpx<T> G = lds[0];
+    for (int v = 1; v < wave; ++v) G = lds[v] + w64 * G;
+    X = X + wt[E * (lane + 1)] * G;
+  }
   __syncthreads();                            // wave totals read
   lds[tid] = X;
   __syncthreads();

@@ -600,6 +600,11 @@ int fp_record_rows(const double* u, const double* rho, const double* rho_s, doub
 }
 
 // ------------------------------------------------------------------ mixed precision
+// The FP32 PCHIP table is read one float4 per interval.
+static bool pcoef_misaligned(const fp_nl_tables_f32* t) {
+  return reinterpret_cast<uintptr_t>(t->pcoef_s) % 16 != 0;
+}
+
 int fp_absmax_f32(const float* x, long long n, unsigned* slot, void* stream) {
   if (!x || !slot || n < 0) return fail(FP_E_ARG, "fp_absmax_f32: bad argument");
   if (reinterpret_cast<uintptr_t>(x) % 16) return fail(FP_E_ARG, "fp_absmax_f32: x must be 16-byte aligned");
@@ -646,6 +651,7 @@ int fp_nl_split_c64(const float* u, const double* rho, const double* rho_s, floa
                     float* alpha, const fp_nl_params* p, const fp_nl_tables_f32* t,
                     void* stream) {
   if (!u || !rho || !rhs || !p || !t) return fail(FP_E_ARG, "fp_nl_split_c64: null pointer");
+  if (pcoef_misaligned(t)) return fail(FP_E_ARG, "fp_nl_split_c64: pcoef_s must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define FP_SPLIT32(NT)                                                                           \
   if (rho_s)                                                                                     \
@@ -669,6 +675,7 @@ int fp_nl_step_c64(const float* in, float* out, const double* rho, const double*
                    int in_spectral, int out_spectral, const fp_nl_params* p,
                    const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream) {
   if (!in || !out || !rho || !p || !t) return fail(FP_E_ARG, "fp_nl_step_c64: null pointer");
+  if (pcoef_misaligned(t)) return fail(FP_E_ARG, "fp_nl_step_c64: pcoef_s must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define FP_STEP32(NT)                                                                            \
   if (rho_s)                                                                                     \
@@ -714,6 +721,10 @@ int fp_batch_plan_create(const fp_batch_instance* inst, int n, fp_batch_plan** p
         !I.nt.Rf || !I.nt.knots || !I.nt.inv_h || !I.nt.pcoef_s || !I.nt.rw ||
         (I.enable_ste && (!I.rho_s || !I.Ws_lut))) {
       snprintf(msg, sizeof msg, "fp_batch_plan_create: instance %d: null pointer", i);
+      return fail(FP_E_ARG, msg);
+    }
+    if (pcoef_misaligned(&I.nt)) {
+      snprintf(msg, sizeof msg, "fp_batch_plan_create: instance %d: pcoef_s must be 16-byte aligned", i);
       return fail(FP_E_ARG, msg);
     }
   }
@@ -801,6 +812,7 @@ int fp_adj_nl_step_c64(const float* v1s, const double* rho, const float* g4s, fl
   if (!v1s || !rho || !g4s || !g1s || !grho || !scratch || !p || !t)
     return fail(FP_E_ARG, "fp_adj_nl_step_c64: null pointer");
   if (p->Nr <= 0) return fail(FP_E_ARG, "fp_adj_nl_step_c64: bad Nr");
+  if (pcoef_misaligned(t)) return fail(FP_E_ARG, "fp_adj_nl_step_c64: pcoef_s must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define FP_ADJNL(NT)                                                                           \
   hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 4>), dim3(p->Nr), dim3(NT / 4), 0, st, v1s, \

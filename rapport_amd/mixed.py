@@ -69,7 +69,7 @@ class MixedTables:
         cs = np.stack([sp.c[0] * h ** 3, sp.c[1] * h ** 2, sp.c[2] * h, sp.c[3]]) * pp
         self.knots = f32(x)
         self.inv_h = f32(1.0 / h)
-        self.pcoef_s = f32(cs)
+        self.pcoef_s = f32(cs.T)                  # (n - 1) x 4, one float4 per interval
         yh, yl, se = split_hankel_fp16(g["Y"])
         self.Yh = torch.from_numpy(yh).to(dev)
         self.Yl = torch.from_numpy(yl).to(dev)
