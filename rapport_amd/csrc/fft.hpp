@@ -140,8 +140,16 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
   }
   if constexpr (!LAST) {
     __syncthreads();
+    if constexpr (TPB % 256 == 0) {
+      // s * TPB leaves bits 0..7 of tid + s * TPB alone, and lds_pad only reads and
+      // flips bits 0..7: one swizzled base per thread, the rest are immediate offsets
+      const int b = lds_pad(tid);
 #pragma unroll
-    for (int s = 0; s < E; ++s) v[s] = lds[lds_pad(tid + s * TPB)];
+      for (int s = 0; s < E; ++s) v[s] = lds[b + s * TPB];
+    } else {
+#pragma unroll
+      for (int s = 0; s < E; ++s) v[s] = lds[lds_pad(tid + s * TPB)];
+    }
     __syncthreads();
     passes<T, N, TPB, INV, Ns * R>(v, lds, tw, tid);
   }

@@ -104,8 +104,14 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   }
   __syncthreads();
   cpx<T> Ps[E];
+  if constexpr (TPB % 256 == 0) {
+    const int b = fpfft::lds_pad(tid);          // s * TPB does not touch the swizzled bits
 #pragma unroll
-  for (int s = 0; s < E; ++s) Ps[s] = lds[fpfft::lds_pad(tid + s * TPB)];
+    for (int s = 0; s < E; ++s) Ps[s] = lds[b + s * TPB];
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) Ps[s] = lds[fpfft::lds_pad(tid + s * TPB)];
+  }
   __syncthreads();
   const cpx<T> wh = wt[NT / 2];
 #pragma unroll
