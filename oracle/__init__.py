@@ -7,13 +7,15 @@ imports, links or executes anything under ``oracle/``. The only callers are
 
 Modules
 -------
-``reference_numpy``  NumPy/SciPy FP64 restatement of ``D3/sim/{config,keldysh,
-                     grids,operators,kernels,integrator,filament_sim}.py``
-                     (``D3`` = ``.github/deliverable (3)``), each function citing
-                     the reference file:line it follows.
-``rate_kernel.c``    the same carrier rate-equation recurrence
-                     (``D3/sim/kernels.py:13-106``) in plain C, built into
-                     ``oracle/_build/librate_oracle.so`` by ``oracle/Makefile``;
-                     used to time the CPU leg of the plasma pass and cross-check
-                     the NumPy loop.
+``reference_numpy``    NumPy/SciPy FP64 restatement of ``D3/sim/{config,keldysh,
+                       grids,operators,kernels,integrator,filament_sim}.py``
+                       (``D3`` = ``.github/deliverable (3)``), each function citing
+                       the reference file:line it follows.
+``adjoint_numpy``      FP64 discrete adjoint of the oracle's forward step (the
+                       reference has no gradients), pinned by finite differences.
+``postprocess_numpy``  the 0-D carrier re-integration (``D3/sim/figures_article.py``)
+                       and the 1-D SPM + shock model (``D3/sim/self_steepening.py``).
+
+The reference path is Python over cupy (no C/C++ sources to compile), so the
+oracle is NumPy/SciPy only.
 """
