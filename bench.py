@@ -279,10 +279,11 @@ def run_adjoint(n_fd=4, h=1e-4, P=16, seed=5):
         fd.append((L64(theta + e) - L64(theta - e)) / (2 * h))
     fd = np.array(fd)
     ga = g[idx]
+    loss64 = L64(theta)        # the FP64 forward's loss at the same input (forward precision)
     return dict(workload="512 x 2048, 1000 steps dz 10 um, 3 uJ 85 fs w0 100 um, 2P = %d LG coefficients" % (2 * P),
                 forward_s=t1 - t0, backward_s=t2 - t1, grad_per_s=1.0 / (t2 - t0),
                 zsteps_per_s_forward=n / (t1 - t0), zsteps_per_s_backward=n / (t2 - t1),
-                loss=float(loss.detach()), fd_check=dict(h=h, coefficients=idx, adjoint=ga.tolist(),
+                loss=float(loss.detach()), loss_fp64_forward=loss64, fd_check=dict(h=h, coefficients=idx, adjoint=ga.tolist(),
                                                            fd_fp64=fd.tolist(),
                                                            rel_l2=float(np.linalg.norm(ga - fd) / np.linalg.norm(fd))))
 
