@@ -26,7 +26,8 @@ restatement of the reference algorithm) on the host for a bounded sample.
 noise, dz 10 um): instance z-steps/s of the graph-captured batched step, the
 time of each pass and the HBM fraction of the non-GEMM passes. `adjoint` times one
 gradient (mixed forward with checkpoints + hand-written adjoint) at 512 x 2048 x 1000
-steps and checks 4 of its 32 components against FP64 central differences.
+steps, checks 4 of its 32 components against FP64 central differences, and runs the
+input-profile fit (L-BFGS on the 32 coefficients towards the FP64 target fluence).
 """
 import argparse
 import json
@@ -232,7 +233,7 @@ ADJ_WORKLOAD = dict(nz=999, Nt=2048, N=513, R_factor=15.0, begin=0.0, end=999 * 
                     save_stride=10 ** 6, rho_t_stride=0, write_output=False, verbose=False)
 
 
-def run_adjoint(n_fd=4, h=1e-4, P=16, seed=5):
+def run_adjoint(n_fd=4, h=1e-4, P=16, seed=5, fit_iters=20):
     """Gradient of the fluence-mismatch loss w.r.t. 2P Laguerre-Gauss coefficients after
     1000 steps at 512 x 2048 (mixed forward + hand-written adjoint, sqrt(n)
     checkpoints), timed; checked on n_fd coefficients against central differences of
@@ -280,10 +281,23 @@ def run_adjoint(n_fd=4, h=1e-4, P=16, seed=5):
     fd = np.array(fd)
     ga = g[idx]
     loss64 = L64(theta)        # the FP64 forward's loss at the same input (forward precision)
+    fit = None
+    if fit_iters > 0:
+        # the use: L-BFGS on the 2P coefficients from theta0 towards the FP64 target fluence
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        th_fit, hist = AD.fit_lg(eng, F_t, torch.tensor(theta0), n, iters=fit_iters, P=P)
+        torch.cuda.synchronize()
+        th_fit = th_fit.cpu().numpy()
+        fit = dict(iters=fit_iters, evaluations=len(hist), seconds=time.perf_counter() - t3,
+                   loss_start=hist[0], loss_end=hist[-1], loss_min=min(hist),
+                   loss_end_fp64_forward=L64(th_fit),
+                   coef_dist_to_hidden_rel=float(np.linalg.norm(th_fit - hidden) /
+                                                 np.linalg.norm(theta0 - hidden)))
     return dict(workload="512 x 2048, 1000 steps dz 10 um, 3 uJ 85 fs w0 100 um, 2P = %d LG coefficients" % (2 * P),
                 forward_s=t1 - t0, backward_s=t2 - t1, grad_per_s=1.0 / (t2 - t0),
                 zsteps_per_s_forward=n / (t1 - t0), zsteps_per_s_backward=n / (t2 - t1),
-                loss=float(loss.detach()), loss_fp64_forward=loss64, fd_check=dict(h=h, coefficients=idx, adjoint=ga.tolist(),
+                loss=float(loss.detach()), loss_fp64_forward=loss64, fit=fit, fd_check=dict(h=h, coefficients=idx, adjoint=ga.tolist(),
                                                            fd_fp64=fd.tolist(),
                                                            rel_l2=float(np.linalg.norm(ga - fd) / np.linalg.norm(fd))))
 
@@ -395,6 +409,8 @@ def main():
                     help="instances of the batched sweep (0: skip the batched measurement)")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--fit-iters", type=int, default=20,
+                    help="L-BFGS iterations of the input-profile fit (0: no fit)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -441,7 +457,7 @@ def main():
     if args.batched > 0:
         out["batched"] = run_batched(args.batched, args.batched_steps)
     if not args.no_adjoint:
-        out["adjoint"] = run_adjoint()
+        out["adjoint"] = run_adjoint(fit_iters=args.fit_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)

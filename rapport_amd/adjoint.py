@@ -257,7 +257,7 @@ def fit_lg(engine, F_target, theta0, n_steps, iters=20, P=None, lr=1.0, history=
         F = fluence_from_input(lg_field(theta, G, E0, tenv), engine, n_steps)
         loss = fluence_loss(F, F_target, w)
         loss.backward()
-        hist.append(float(loss))
+        hist.append(float(loss.detach()))
         return loss
     opt.step(closure)
     return theta.detach(), hist
