@@ -762,8 +762,8 @@ PINS = {
     "LinearOperator": "Gaussian beam q-parameter diffraction, chirped-pulse dispersion, energy",
     "NonlinearOperator.split": "toggles-off => rhs=alpha=0; Kerr-only => i kappa |u|^2 u; "
                                "photo/alpha pointwise closed forms; T-hat off => T=1",
-    "rate_equation_rows": "constant-I exact solution of dx/dt=S+Lx; C restatement; "
-                          "non-negativity and rho<=rho_max",
+    "rate_equation_rows": "constant-I exact solution of dx/dt=S+Lx; independent scalar "
+                          "restatement; non-negativity and rho<=rho_max",
     "Integrator": "linear-only run = linear Gaussian propagation; nz+1 steps, z labels; "
                   "record invariants (bins sum, rho_at<=rho_rz, max Ipeak = max Imax); "
                   "result.npz size estimate 0.99 GB / 86% cube at the 1030 nm config",

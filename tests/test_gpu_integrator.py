@@ -117,3 +117,46 @@ def test_graph_replay_matches_eager(precision):
     assert torch.equal(a.u, b.u)
     assert torch.equal(a.rho, b.rho)
     assert float(junk[0]) == 4.0
+
+
+# params.json fields written by the reference (D3/sim/integrator.py:305-340)
+PARAMS_KEYS = {"n0", "n0_probe", "nc_probe_cm3", "lambda_probe_nm", "wavelength_nm", "n2", "U_g_eV",
+               "Us_eV", "energy_uJ", "w0_um", "delta_t_fs", "code_fingerprint", "begin_um", "end_um",
+               "z_focus_air_um", "toggles"}
+TOGGLE_KEYS = {"enable_kerr_instantaneous", "enable_kerr_raman", "enable_self_steepening",
+               "enable_photoionization_loss", "enable_plasma_defocusing", "enable_plasma_absorption",
+               "enable_space_time_focusing", "enable_spectral_filter", "tau_ste_fs", "enable_ste_index",
+               "E_tr_eV", "f_ste_pump", "enable_avalanche", "enable_recombination", "enable_ste"}
+
+
+def test_run_writes_reference_outputs(tmp_path):
+    """run() through the reference's entry point with its defaults (recording every step,
+    cube subsampling, STE on): result.npz holds the same keys and values as the oracle's,
+    params.json the reference's fields."""
+    import json
+    import rapport_amd as R
+    _announce("run() writes result.npz / params.json")
+    kw = dict(Nz=6, Nt=512, Nr=129, wavelength=800e-9, energy_uJ=20.0, w0=30e-6, delta_t=85e-15,
+              begin=0.0, end=60e-6, n2=3.2e-20, R_factor=20.0, verbose=False, t_window=800e-15,
+              rho_t_stride=15)
+    res_p = R.run(out_dir=str(tmp_path / "gpu"), **kw)
+    res_o = O.run(out_dir=str(tmp_path / "cpu"), **kw)
+    zp = np.load(tmp_path / "gpu" / "result.npz", allow_pickle=True)      # written by this test
+    zo = np.load(tmp_path / "cpu" / "result.npz", allow_pickle=True)
+    assert set(zp.files) == set(zo.files) == set(res_o)
+    for k in zo.files:
+        a, b = zp[k], zo[k]
+        if b.dtype == object:                       # None entries (disabled outputs)
+            assert a.dtype == object and a.item() is None and b.item() is None, k
+            continue
+        assert a.shape == b.shape, k
+        if np.abs(b).max() == 0:
+            assert np.abs(a).max() == 0, k
+        else:
+            assert rel_l2(a, b) <= 1e-9, (k, rel_l2(a, b))
+        np.testing.assert_array_equal(a, res_p[k] if res_p[k] is not None else a)
+    assert res_o["rho_rzt"] is not None and res_o["rho_s_rzt"] is not None   # the cube is on
+    params = json.load(open(tmp_path / "gpu" / "params.json"))
+    assert set(params) == PARAMS_KEYS and set(params["toggles"]) == TOGGLE_KEYS
+    assert params["toggles"]["enable_ste"] is True
+    assert abs(params["n0"] - json.load(open(tmp_path / "cpu" / "params.json"))["n0"]) == 0.0
