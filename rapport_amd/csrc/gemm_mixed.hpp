@@ -14,9 +14,11 @@
 //   r = row in block; element j of lane (kg*16 + r) is Y[16 mb + r][32 ks + 8 kg + j].
 // B is split on the fly while staging into LDS in the same fragment order.
 //
-// Tile BM x BN per 256-thread block (2 x 2 waves, each (BM/2) x (BN/2)) with
-// BM, BN in {64, 128}; K slab 32, LDS double-buffered, two blocks per CU. The
-// epilogue also reduces max|C| into a device word for the next GEMM in the chain.
+// Tile BM x BN per block of WM x WN waves, each (BM/WM) x (BN/WN): 2 x 2 waves with
+// BM, BN in {64, 128} (or 128 x 256), two blocks per CU; or 4 x 2 waves of 64 x 64 for a
+// 256 x 128 tile, one block per CU (each B column panel is then split by half as many row
+// tiles). K slab 32, LDS double-buffered. The epilogue also reduces max|C| into a device
+// word for the next GEMM in the chain.
 #pragma once
 #include "common.hpp"
 
@@ -41,15 +43,15 @@ FP_DEV int scale_exp(float mx) {
 }
 
 // One K slab of a BM-row (BN-column) tile is BM * 4 h8 fragments.
-template <int BM, int BN> struct Smem {
+template <int BM, int BN, int WM = 2, int WN = 2> struct Smem {
   h8 Ah[2][BM * 4], Al[2][BM * 4];
   h8 Bh[2][BN * 4], Bl[2][BN * 4];
-  float red[NTHR / 64];
+  float red[WM * WN];
 };
 
 // One BM x BN output tile (tm, tn) of C = alpha diag(rowscale) Y B (x phase).
-template <int BM, int BN>
-FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Yh,
+template <int BM, int BN, int WM = 2, int WN = 2>
+FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __restrict__ Yh,
                       const h8* __restrict__ Yl, int y_scale_exp, const float* __restrict__ B,
                       float* __restrict__ C, int M, int K, int Ncols, float alpha,
                       const float* __restrict__ rowscale, const float* __restrict__ phase,
@@ -57,11 +59,13 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
   const int tid = threadIdx.x;
   const int m0 = tm * BM, n0 = tn * BN;
   const int lane = tid & 63, wave = tid >> 6;
-  constexpr int FA = BM / 32, FB = BN / 32;       // 16-row / 16-col blocks per wave
-  constexpr int CPT = NTHR / BN;                   // threads per B column
+  constexpr int NT = WM * WN * 64;
+  constexpr int FA = BM / WM / 16, FB = BN / WN / 16;   // 16-row / 16-col blocks per wave
+  constexpr int CPT = NT / BN;                     // threads per B column
   constexpr int GPT = 4 / CPT;                     // k groups of 8 staged per thread
-  constexpr int APT = BM * 4 / NTHR;               // A fragments staged per thread
-  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+  constexpr int APT = BM * 4 / NT;                 // A fragments staged per thread
+  static_assert(GPT >= 1 && APT >= 1, "tile too small for the block");
+  const int wm = (wave / WN) * (BM / WM), wn = (wave % WN) * (BN / WN);
 
   const int sb_exp = scale_exp(__uint_as_float(*bmax_in));
   const float sB = pow2f(sb_exp);
@@ -100,8 +104,8 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
     const unsigned base = 16u * ((unsigned)(ks * a_slab_stride + a_tile_off) + (unsigned)tid);
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
-      R.ra_h[i] = *reinterpret_cast<const h8*>(Yhc + (base + 16u * (unsigned)(i * NTHR)));
-      R.ra_l[i] = *reinterpret_cast<const h8*>(Ylc + (base + 16u * (unsigned)(i * NTHR)));
+      R.ra_h[i] = *reinterpret_cast<const h8*>(Yhc + (base + 16u * (unsigned)(i * NT)));
+      R.ra_l[i] = *reinterpret_cast<const h8*>(Ylc + (base + 16u * (unsigned)(i * NT)));
     }
   };
   auto sstore = [&](int buf, const Stage& R) {
@@ -122,8 +126,8 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
     }
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
-      sm.Ah[buf][tid + i * NTHR] = R.ra_h[i];
-      sm.Al[buf][tid + i * NTHR] = R.ra_l[i];
+      sm.Ah[buf][tid + i * NT] = R.ra_h[i];
+      sm.Al[buf][tid + i * NT] = R.ra_l[i];
     }
   };
 
@@ -226,7 +230,7 @@ FP_DEV void gemm_tile(Smem<BM, BN>& sm, int tm, int tn, const h8* __restrict__ Y
     __syncthreads();
     if (tid == 0) {
       float m = sm.red[0];
-      for (int w = 1; w < NTHR / 64; ++w) m = fmaxf(m, sm.red[w]);
+      for (int w = 1; w < NT / 64; ++w) m = fmaxf(m, sm.red[w]);
       atomicMax(cmax_out, __float_as_uint(m));
     }
   }
@@ -242,18 +246,18 @@ FP_DEV int xcd_remap(int id, int nb) {
 // (one block per CU, up to 512 VGPR + AGPR per lane), the smaller ones two.
 template <int BM, int BN> struct TileOcc { static constexpr int value = (BM * BN >= 128 * 256) ? 1 : 2; };
 
-template <int BM, int BN>
-__global__ __launch_bounds__(NTHR, (TileOcc<BM, BN>::value)) void hankel_gemm_f16x3(
+template <int BM, int BN, int WM = 2, int WN = 2>
+__global__ __launch_bounds__(WM * WN * 64, (WM * WN > 4 ? 1 : TileOcc<BM, BN>::value)) void hankel_gemm_f16x3(
     const h8* __restrict__ Yh, const h8* __restrict__ Yl, int y_scale_exp,
     const float* __restrict__ B, float* __restrict__ C, int M, int K, int Ncols, float alpha,
     const float* __restrict__ rowscale, const float* __restrict__ phase,
     const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out,
     unsigned* __restrict__ reset_slot) {
-  __shared__ Smem<BM, BN> sm;
+  __shared__ Smem<BM, BN, WM, WN> sm;
   if (reset_slot && blockIdx.x == 0 && threadIdx.x == 0) *reset_slot = 0u;
   const int tilesM = M / BM, tilesN = Ncols / BN;
   const int lin = xcd_remap(blockIdx.x, tilesM * tilesN);
-  gemm_tile<BM, BN>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
+  gemm_tile<BM, BN, WM, WN>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
             rowscale, phase, bmax_in, cmax_out);
 }
 

@@ -637,13 +637,21 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
   const int Ncols = 2 * Nt;
   if (Nr <= 0 || Nr % 128 || Ncols % 128)
     return fail(FP_E_SHAPE, "fp_hankel_gemm_mixed: needs Nr % 128 == 0 and 2 Nt % 128 == 0");
-  // 128 x 128 tiles, two blocks per CU (a 128 x 256 tile at one block per CU measured
-  // slower at 1024 x 4096: 85 vs 66 us, latency no longer hidden).
+  // 256 x 128 tiles of 8 waves (4 x 2 of 64 x 64), one block per CU, when Nr allows: 5 %
+  // faster than 128 x 128 tiles of 4 waves at 1024 x 4096 (each B panel is staged by half
+  // as many blocks), results bitwise equal; else 128 x 128, two blocks per CU. (A 128 x 256
+  // tile of 4 waves at one block per CU measured slower: 85 vs 66 us.)
   const auto* yh = reinterpret_cast<const fpgemm16::h8*>(Yh);
   const auto* yl = reinterpret_cast<const fpgemm16::h8*>(Yl);
-  hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<128, 128>), dim3((Nr / 128) * (Ncols / 128)),
-                     dim3(fpgemm16::NTHR), 0, S(stream), yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols,
-                     alpha, rowscale, phase, bmax_in, cmax_out, reset_slot);
+  if (Nr % 256 == 0)
+    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<256, 128, 4, 2>),
+                       dim3((Nr / 256) * (Ncols / 128)), dim3(512), 0, S(stream), yh, yl,
+                       y_scale_exp, B, C, Nr, Nr, Ncols, alpha, rowscale, phase, bmax_in, cmax_out,
+                       reset_slot);
+  else
+    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<128, 128>), dim3((Nr / 128) * (Ncols / 128)),
+                       dim3(256), 0, S(stream), yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols, alpha,
+                       rowscale, phase, bmax_in, cmax_out, reset_slot);
   return check_launch("hankel_gemm_mixed");
 }
 
