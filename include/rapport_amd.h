@@ -288,9 +288,9 @@ enum { FP_PASS_PLASMA = 0, FP_PASS_G1 = 1, FP_PASS_G2 = 2, FP_PASS_NL = 3, FP_PA
 int fp_batch_plan_create(const fp_batch_instance* inst, int n, fp_batch_plan** plan);
 int fp_batch_plan_destroy(fp_batch_plan* plan);
 
-/* Launch configuration of instance i: GEMM tile bm x bn (each 64 or 128; Nr % bm == 0,
- * 2 Nt % bn == 0) and nonlinear threads per row nl_tpb (Nt/8 or Nt/4); 0 keeps the
- * current value. */
+/* Launch configuration of instance i: GEMM tile bm x bn (64/128 x 64/128 with 4 waves,
+ * 128 x 256 with 4 waves, or 256 x 128 with 8 waves; Nr % bm == 0, 2 Nt % bn == 0) and
+ * nonlinear threads per row nl_tpb (Nt/8 or Nt/4); 0 keeps the current value. */
 int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb);
 
 /* Grid of the launches of pass kind 0 = plasma, 1 = GEMM, 2 = nonlinear: 0 = one

@@ -3,7 +3,8 @@
 Searched per shape class, on the live batch, timing each candidate with HIP
 events on the launch stream (median of ``reps`` after ``warmup`` launches):
 
-* GEMM tile bm x bn in {64, 128}^2 or 128 x 256 per (Nr, Nt): timed on the G2 pass (w1 -> w0,
+* GEMM tile bm x bn in {64, 128}^2, 128 x 256 or 256 x 128 (8 waves) per (Nr, Nt): timed on
+  the G2 pass (w1 -> w0,
   no phase), which reads and writes scratch buffers only;
 * nonlinear threads per row, Nt/8 or Nt/4, per (Nt, STE): timed on the
   nonlinear pass, which works in place on the scratch buffer w0;
@@ -27,7 +28,7 @@ import torch
 from . import _lib
 from ._lib import call, stream_ptr
 
-TILES = [(128, 256), (128, 128), (128, 64), (64, 128), (64, 64)]
+TILES = [(256, 128), (128, 256), (128, 128), (128, 64), (64, 128), (64, 64)]
 GRIDS = [0, 256, 512, 1024, 2048]
 
 

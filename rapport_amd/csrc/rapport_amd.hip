@@ -273,10 +273,11 @@ __global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void batch_nl_kernel(
 // single-instance sequence). The stage is a template parameter: a run-time switch
 // selecting the output pointer was miscompiled by hipcc (ROCm 7.2) -- its default
 // case kept case 1's pointer.
-template <int BM, int BN, int STAGE>
-__global__ __launch_bounds__(fpgemm16::NTHR, (fpgemm16::TileOcc<BM, BN>::value)) void batch_gemm_kernel(
-    const fp_batch_instance* __restrict__ insts, BatchGroup g) {
-  __shared__ fpgemm16::Smem<BM, BN> sm;
+template <int BM, int BN, int STAGE, int WM = 2, int WN = 2>
+__global__ __launch_bounds__(WM * WN * 64, (WM * WN > 4 ? 1 : fpgemm16::TileOcc<BM, BN>::value))
+void batch_gemm_kernel(const fp_batch_instance* __restrict__ insts, BatchGroup g) {
+  constexpr int NT = WM * WN * 64;
+  __shared__ fpgemm16::Smem<BM, BN, WM, WN> sm;
   const int total = g.begin[g.n];
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     // XCD-aware order also on a persistent grid: with gridDim.x % 8 == 0, iteration `it`
@@ -302,14 +303,14 @@ __global__ __launch_bounds__(fpgemm16::NTHR, (fpgemm16::TileOcc<BM, BN>::value))
       // zero input columns in G2 / G4); write the zeros, skip the MFMA work
       float4* Cz = reinterpret_cast<float4*>(C);
       const int ld4 = (2 * I.Nt) / 4;
-      for (int i = threadIdx.x; i < BM * BN / 4; i += fpgemm16::NTHR) {
+      for (int i = threadIdx.x; i < BM * BN / 4; i += NT) {
         const int r = i / (BN / 4), c4 = i % (BN / 4);
         Cz[(size_t)(tm * BM + r) * ld4 + (tn * BN) / 4 + c4] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
       __syncthreads();
       continue;
     }
-    fpgemm16::gemm_tile<BM, BN>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(I.Yh),
+    fpgemm16::gemm_tile<BM, BN, WM, WN>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(I.Yh),
                                 reinterpret_cast<const fpgemm16::h8*>(I.Yl), I.y_scale_exp, B, C,
                                 I.Nr, I.Nr, 2 * I.Nt, alpha, rs, ph, bin, cout);
     __syncthreads();
@@ -446,23 +447,24 @@ int launch_batch_nl(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipS
   return check_launch("batch_nl_step");
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int WM = 2, int WN = 2>
 void launch_batch_gemm_t(const fp_batch_plan& P, const fp_batch_plan::Launch& L, int stage,
                          hipStream_t st) {
   const dim3 grid = grid_of(L.g, P.grid[1]);
-  const dim3 blk(fpgemm16::NTHR);
+  const dim3 blk(WM * WN * 64);
   switch (stage) {
-    case 0: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 0>), grid, blk, 0, st, P.dev, L.g); break;
-    case 1: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 1>), grid, blk, 0, st, P.dev, L.g); break;
-    case 2: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 2>), grid, blk, 0, st, P.dev, L.g); break;
-    default: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 3>), grid, blk, 0, st, P.dev, L.g); break;
+    case 0: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 0, WM, WN>), grid, blk, 0, st, P.dev, L.g); break;
+    case 1: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 1, WM, WN>), grid, blk, 0, st, P.dev, L.g); break;
+    case 2: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 2, WM, WN>), grid, blk, 0, st, P.dev, L.g); break;
+    default: hipLaunchKernelGGL((batch_gemm_kernel<BM, BN, 3, WM, WN>), grid, blk, 0, st, P.dev, L.g); break;
   }
 }
 
 int launch_batch_gemm(const fp_batch_plan& P, const fp_batch_plan::Launch& L, int stage,
                       hipStream_t st) {
   if (stage < 0 || stage > 3) return fail(FP_E_ARG, "batch GEMM: bad stage");
-  if (L.a == 128 && L.b == 256) launch_batch_gemm_t<128, 256>(P, L, stage, st);
+  if (L.a == 256 && L.b == 128) launch_batch_gemm_t<256, 128, 4, 2>(P, L, stage, st);
+  else if (L.a == 128 && L.b == 256) launch_batch_gemm_t<128, 256>(P, L, stage, st);
   else if (L.a == 128 && L.b == 128) launch_batch_gemm_t<128, 128>(P, L, stage, st);
   else if (L.a == 128 && L.b == 64) launch_batch_gemm_t<128, 64>(P, L, stage, st);
   else if (L.a == 64 && L.b == 128) launch_batch_gemm_t<64, 128>(P, L, stage, st);
@@ -772,10 +774,11 @@ int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb) 
   const fp_batch_instance& I = plan->host[i];
   const int m = bm ? bm : plan->bm[i], c = bn ? bn : plan->bn[i];
   const int t = nl_tpb ? nl_tpb : plan->nl_tpb[i];
-  const bool tile_ok = (m == 64 || m == 128) && (c == 64 || c == 128 || (c == 256 && m == 128));
+  const bool tile_ok = ((m == 64 || m == 128) && (c == 64 || c == 128)) ||
+                       (m == 128 && c == 256) || (m == 256 && c == 128);
   if (!tile_ok || I.Nr % m || (2 * I.Nt) % c)
-    return fail(FP_E_SHAPE, "fp_batch_set_config: tile must be 64/128 x 64/128 or 128 x 256 and "
-                            "divide Nr x 2 Nt");
+    return fail(FP_E_SHAPE, "fp_batch_set_config: tile must be 64/128 x 64/128, 128 x 256 or "
+                            "256 x 128 and divide Nr x 2 Nt");
   if (t != I.Nt / 8 && t != I.Nt / 4)
     return fail(FP_E_ARG, "fp_batch_set_config: nonlinear threads must be Nt/8 or Nt/4");
   plan->bm[i] = m;

@@ -204,6 +204,28 @@ def test_batch_tiles_and_persistent_grid_bitwise(batch5, tile):
         assert torch.equal(r, r1), f"instance {k}: rho differs"
 
 
+def test_batch_eight_wave_tile_bitwise(batch5):
+    """256 x 128 tiles of 8 waves on the instances whose Nr allows them, the default tile on
+    the others: the same per-element arithmetic, so the state equals the default run's."""
+    from rapport_amd.batch import BatchedIntegrator
+    _announce("batch GEMM tile 256 x 128 (8 waves), 5 steps")
+    cfgs, envs = _cfgs()
+    b = BatchedIntegrator(cfgs, envs)
+    used = 0
+    for i, m in enumerate(b.members):
+        if m.Nr % 256 == 0:
+            b.set_config(i, 256, 128, 0)
+            used += 1
+    assert used >= 1
+    for _ in range(5):
+        b.step()
+    torch.cuda.synchronize()
+    for k, ((s, r), (s1, r1)) in enumerate(zip(_state(b), batch5)):
+        assert torch.equal(s, s1), f"instance {k}: spec differs, rel {rel_l2(s.cpu(), s1.cpu()):.2e}"
+        assert torch.equal(r, r1), f"instance {k}: rho differs"
+    b.close()
+
+
 def test_batch_wide_nl_close(singles5):
     from rapport_amd.batch import BatchedIntegrator
     _announce("batch nonlinear Nt/4 threads, 5 steps")
