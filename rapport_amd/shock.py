@@ -52,7 +52,10 @@ def propagate_shock_batch(t_list, E0_rows, z_max, nz, lam0=800e-9, n0=1.4533, n2
     dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
     if dev.type != "cuda":
         raise ValueError("propagate_shock_batch runs on the HIP device")
-    E0 = torch.as_tensor(np.asarray(E0_rows), dtype=torch.complex128, device=dev)
+    if torch.is_tensor(E0_rows):
+        E0 = E0_rows.to(device=dev, dtype=torch.complex128)
+    else:
+        E0 = torch.as_tensor(np.asarray(E0_rows), dtype=torch.complex128, device=dev)
     if E0.dim() == 1:
         E0 = E0[None, :]
     E0 = E0.contiguous()
