@@ -69,12 +69,13 @@ def _integrator(precision, save_stride=10 ** 9):
 
 
 def run_gpu(precision, steps, warmup):
-    """Times `steps` pipeline steps twice: first as plain launches (the throughput), then
-    with HIP event pairs bracketing every Hankel GEMM launch (its duration for the
-    roofline; the event records themselves slow the step by about 5 %)."""
+    """Times `steps` pipeline steps three times: first as plain launches (the throughput),
+    then with HIP event pairs bracketing every Hankel GEMM launch (its duration for the
+    roofline; the event records themselves slow the step by about 5 %), then with one
+    record of every history after each step (the reference's default save_stride = 1)."""
     import torch
     import rapport_amd._lib as L
-    it = _integrator(precision)
+    it = _integrator(precision, save_stride=1)
     for _ in range(warmup):
         it._pipeline_step(it.dz)
     torch.cuda.synchronize()
@@ -111,9 +112,20 @@ def run_gpu(precision, steps, warmup):
         elapsed_ev = time.perf_counter() - t1
     finally:
         integ.call, mixed.call = orig, orig
+    it.k_save = 0
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for i in range(steps):
+        it._pipeline_step(it.dz)
+        if it.k_save == it.n_saves:
+            it.k_save = 0
+        it._record(i * it.dz)
+    torch.cuda.synchronize()
+    elapsed_rec = time.perf_counter() - t2
     gemm_ms = sum(a.elapsed_time(b) for a, b in gemm_ev) / max(len(gemm_ev), 1)
     return dict(elapsed=elapsed, steps_per_s=steps / elapsed, ms_per_step=1e3 * elapsed / steps,
                 ms_per_step_gemm_events=1e3 * elapsed_ev / steps,
+                ms_per_step_recording=1e3 * elapsed_rec / steps,
                 gemm_ms=gemm_ms, n_gemm=len(gemm_ev), capped_rows=int(it.t.status.item()),
                 Nr=it.Nr, Nt=it.Nt)
 
@@ -443,10 +455,14 @@ def main():
         roofline=roof, roofline_nongemm=ng,
         fp64=dict(value=runs["fp64"]["steps_per_s"], ms_per_step=runs["fp64"]["ms_per_step"],
                   ms_per_step_gemm_events=runs["fp64"]["ms_per_step_gemm_events"],
-                  gemm_ms=runs["fp64"]["gemm_ms"]) if "fp64" in runs else None,
+                  gemm_ms=runs["fp64"]["gemm_ms"],
+                  ms_per_step_recording_every_step=runs["fp64"]["ms_per_step_recording"])
+        if "fp64" in runs else None,
         mixed=dict(value=runs["mixed"]["steps_per_s"], ms_per_step=runs["mixed"]["ms_per_step"],
                    ms_per_step_gemm_events=runs["mixed"]["ms_per_step_gemm_events"],
-                   gemm_ms=runs["mixed"]["gemm_ms"]) if "mixed" in runs else None,
+                   gemm_ms=runs["mixed"]["gemm_ms"],
+                   ms_per_step_recording_every_step=runs["mixed"]["ms_per_step_recording"])
+        if "mixed" in runs else None,
         mixed_speedup_vs_fp64=(runs["mixed"]["steps_per_s"] / runs["fp64"]["steps_per_s"]
                                if len(runs) == 2 else None),
         accuracy_mixed_vs_fp64=acc,

@@ -166,6 +166,45 @@ int fp_record_rows(const double* u, const double* rho, const double* rho_s, doub
                    const fp_nl_params* p, const fp_nl_tables* t, double dt, int nbins,
                    const int* bin_lo, const int* bin_hi, double* bins, void* stream);
 
+/* Destinations of one record of Integrator._record (D3/sim/integrator.py:179-224):
+ * the entries at the record's index of each history, float32 unless noted.
+ * Row vectors hold Nr values, on-axis rows Nt, the cube Nr_sub x Nt_sub with
+ * Nr_sub = (Nr-1)/r_stride + 1, Nt_sub = (Nt-1)/t_stride + 1; bins is nbins x Nr. */
+typedef struct fp_record_dst {
+  float* fluence;       /* sum_t |u|^2 dt invE2 */
+  float* rho;           /* max_t rho */
+  float* rho_s;         /* max_t rho_s */
+  float* ipeak;         /* max_t I */
+  float* absorbed;      /* sum_t dep */
+  float* bins;          /* sum_t dep over [bin_lo[b], bin_hi[b]); NULL iff nbins == 0 */
+  float* rho_at;        /* rho[:, it_snap]; NULL to skip */
+  int it_snap;
+  float* rho_onaxis;    /* rho[0, :] */
+  float* rho_s_onaxis;  /* rho_s[0, :] */
+  float* I_onaxis;      /* I[0, :] */
+  int r_stride, t_stride, Nt_sub;  /* t_stride = 0 skips the cube */
+  float* rho_cube;      /* rho[::r_stride, ::t_stride] */
+  float* rho_s_cube;
+  float* I_cube;
+  const double* r_w;    /* Nr radial quadrature weights 2 pi r dr */
+  double* partial;      /* 3 x Nr FP64 scratch */
+  double* imax;         /* FP64: max_r max_t I */
+  double* dedz_photo;   /* FP64: sum_r (sum_t 2 photo I) dt r_w 1e10 */
+  double* dedz_plasma;  /* FP64: sum_r (sum_t 2 plasma I) dt r_w 1e10 */
+} fp_record_dst;
+
+/* One whole record from the spectral state `spec` (Nr x Nt, complex128 when
+ * spec_c64 == 0 with the FP64 twiddles `tw`, else complex64 with the FP32
+ * twiddles): the inverse transform along t and the reductions of
+ * fp_record_rows, written straight into the histories of `d`. Replaces the
+ * inverse fp_fft_rows + fp_record_rows + per-history copies of
+ * Integrator._record (D3/sim/integrator.py:179-224). rho_s may be NULL (read
+ * as zero). */
+int fp_record_history(const void* spec, int spec_c64, const void* tw, const double* rho,
+                      const double* rho_s, const fp_nl_params* p, const fp_nl_tables* t,
+                      double dt, int nbins, const int* bin_lo, const int* bin_hi,
+                      const fp_record_dst* d, void* stream);
+
 /* ===========================================================================
  * Mixed-precision path (Config.precision = "mixed"): complex64 fields, FP32
  * transforms and RK4, split-FP16 Hankel GEMM with FP32 accumulation, FP64

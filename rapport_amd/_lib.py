@@ -63,6 +63,15 @@ class Rate0DParams(ctypes.Structure):
                 ("Imax", c_double)]
 
 
+class RecordDst(ctypes.Structure):
+    """fp_record_dst (history entries of one record)."""
+    _fields_ = [("fluence", P), ("rho", P), ("rho_s", P), ("ipeak", P), ("absorbed", P),
+                ("bins", P), ("rho_at", P), ("it_snap", c_int), ("rho_onaxis", P),
+                ("rho_s_onaxis", P), ("I_onaxis", P), ("r_stride", c_int), ("t_stride", c_int),
+                ("Nt_sub", c_int), ("rho_cube", P), ("rho_s_cube", P), ("I_cube", P), ("r_w", P),
+                ("partial", P), ("imax", P), ("dedz_photo", P), ("dedz_plasma", P)]
+
+
 # name -> argtypes (restype is int for all but fp_last_error)
 _SIGS = {
     "fp_version": [],
@@ -75,6 +84,8 @@ _SIGS = {
     "fp_nl_step": [P, P, P, P, c_int, c_int, ctypes.POINTER(NLParams), ctypes.POINTER(NLTables), P],
     "fp_record_rows": [P, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTables), c_double,
                        c_int, P, P, P, P],
+    "fp_record_history": [P, c_int, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTables),
+                          c_double, c_int, P, P, ctypes.POINTER(RecordDst), P],
     "fp_absmax_f32": [P, ctypes.c_longlong, P, P],
     "fp_fft_rows_c64": [P, P, c_int, c_int, c_int, P, P],
     "fp_plasma_scan_spec_c64": [P, P, P, P, ctypes.POINTER(PlasmaParams), P, P, P, P, P, P],

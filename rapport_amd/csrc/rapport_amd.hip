@@ -20,6 +20,7 @@
 #include "nl_f32.hpp"
 #include "plasma.hpp"
 #include "rate0d.hpp"
+#include "record.hpp"
 #include "shock1d.hpp"
 
 #define FP_VERSION 100
@@ -494,6 +495,33 @@ int run_pass(fp_batch_plan& P, int pass, hipStream_t st) {
 
 bool valid_nt(int Nt) { return Nt == 512 || Nt == 1024 || Nt == 2048 || Nt == 4096; }
 
+// The row kernel runs with fft_rows_kernel's block size for each Nt, so its
+// time-domain row is the one fp_fft_rows returns.
+template <typename T, int NT, int TPB>
+int launch_record_history(const void* spec, const void* tw, const double* rho, const double* rho_s,
+                          const fp_nl_params& p, const fp_nl_tables& t, double dt, int nbins,
+                          const int* bin_lo, const int* bin_hi, const fp_record_dst& d,
+                          hipStream_t st) {
+  hipLaunchKernelGGL((fprec::record_history_kernel<T, NT, TPB>), dim3(p.Nr), dim3(TPB), 0, st,
+                     static_cast<const T*>(spec), static_cast<const T*>(tw), rho, rho_s, p, t, dt,
+                     nbins, bin_lo, bin_hi, d);
+  return check_launch("record_history");
+}
+
+template <typename T>
+int dispatch_record_history(const void* spec, const void* tw, const double* rho,
+                            const double* rho_s, const fp_nl_params& p, const fp_nl_tables& t,
+                            double dt, int nbins, const int* bin_lo, const int* bin_hi,
+                            const fp_record_dst& d, hipStream_t st) {
+  switch (p.Nt) {
+    case 512: return launch_record_history<T, 512, 64>(spec, tw, rho, rho_s, p, t, dt, nbins, bin_lo, bin_hi, d, st);
+    case 1024: return launch_record_history<T, 1024, 128>(spec, tw, rho, rho_s, p, t, dt, nbins, bin_lo, bin_hi, d, st);
+    case 2048: return launch_record_history<T, 2048, 256>(spec, tw, rho, rho_s, p, t, dt, nbins, bin_lo, bin_hi, d, st);
+    case 4096: return launch_record_history<T, 4096, 256>(spec, tw, rho, rho_s, p, t, dt, nbins, bin_lo, bin_hi, d, st);
+    default: return fail(FP_E_SHAPE, "fp_record_history: Nt must be 512, 1024, 2048 or 4096");
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -599,6 +627,40 @@ int fp_record_rows(const double* u, const double* rho, const double* rho_s, doub
   hipLaunchKernelGGL((fpnl::record_kernel<256>), dim3(p->Nr), dim3(256), 0, S(stream), u, rho,
                      rho_s, rows_out, *p, *t, dt, nbins, bin_lo, bin_hi, bins);
   return check_launch("record_rows");
+}
+
+int fp_record_history(const void* spec, int spec_c64, const void* tw, const double* rho,
+                      const double* rho_s, const fp_nl_params* p, const fp_nl_tables* t,
+                      double dt, int nbins, const int* bin_lo, const int* bin_hi,
+                      const fp_record_dst* d, void* stream) {
+  if (!spec || !tw || !rho || !p || !t || !d)
+    return fail(FP_E_ARG, "fp_record_history: null pointer");
+  if (p->Nr <= 0) return fail(FP_E_SHAPE, "fp_record_history: Nr must be positive");
+  if (!d->fluence || !d->rho || !d->rho_s || !d->ipeak || !d->absorbed || !d->rho_onaxis ||
+      !d->rho_s_onaxis || !d->I_onaxis || !d->r_w || !d->partial || !d->imax ||
+      !d->dedz_photo || !d->dedz_plasma)
+    return fail(FP_E_ARG, "fp_record_history: null destination");
+  if (nbins < 0 || (nbins > 0 && (!bin_lo || !bin_hi || !d->bins)))
+    return fail(FP_E_ARG, "fp_record_history: bins");
+  if (d->rho_at && (d->it_snap < 0 || d->it_snap >= p->Nt))
+    return fail(FP_E_ARG, "fp_record_history: it_snap outside [0, Nt)");
+  if (d->t_stride < 0) return fail(FP_E_ARG, "fp_record_history: t_stride < 0");
+  if (d->t_stride > 0) {
+    if (d->r_stride <= 0) return fail(FP_E_ARG, "fp_record_history: r_stride must be positive");
+    if (d->Nt_sub != (p->Nt - 1) / d->t_stride + 1)
+      return fail(FP_E_ARG, "fp_record_history: Nt_sub != (Nt-1)/t_stride + 1");
+    if (!d->rho_cube || !d->rho_s_cube || !d->I_cube)
+      return fail(FP_E_ARG, "fp_record_history: null cube destination");
+  }
+  const hipStream_t st = S(stream);
+  const int rc = spec_c64 ? dispatch_record_history<float>(spec, tw, rho, rho_s, *p, *t, dt, nbins,
+                                                           bin_lo, bin_hi, *d, st)
+                          : dispatch_record_history<double>(spec, tw, rho, rho_s, *p, *t, dt,
+                                                            nbins, bin_lo, bin_hi, *d, st);
+  if (rc != FP_OK) return rc;
+  hipLaunchKernelGGL(fprec::record_finish_kernel, dim3(1), dim3(fprec::FIN_TPB), 0, st, p->Nr, dt,
+                     *d);
+  return check_launch("record_finish");
 }
 
 // ------------------------------------------------------------------ mixed precision

@@ -32,7 +32,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import call, ptr, stream_ptr
+from ._lib import RecordDst, call, ptr, stream_ptr
 from .config import Config, code_fingerprint
 from .grids import ENVELOPES, build_grids
 from .operators import DeviceTables, LinearOperator, NonlinearOperator
@@ -101,6 +101,7 @@ class Integrator:
         self._r_stride = max(1, cfg.rho_r_stride)
         if self._t_stride > 0:
             Nt_sub = (self.Nt - 1) // self._t_stride + 1
+            self._Nt_sub = Nt_sub
             Nr_sub = (self.Nr - 1) // self._r_stride + 1
             self.rho_rzt = torch.zeros(n_saves, Nr_sub, Nt_sub, **f32)
             self.rho_s_rzt = torch.zeros(n_saves, Nr_sub, Nt_sub, **f32)
@@ -135,7 +136,6 @@ class Integrator:
             self._bin_hi = torch.tensor(hi, dtype=torch.int32, device=dev)
             self.absorb_bin_edges_fs = edges
             self.absorbed_rz_bins = torch.zeros(n_saves, len(lo), self.Nr, **f32)
-            self._bins_tmp = torch.zeros(len(lo), self.Nr, dtype=torch.float64, device=dev)
         else:
             self._bin_lo = self._bin_hi = None
             self.absorb_bin_edges_fs = self.absorbed_rz_bins = None
@@ -149,7 +149,7 @@ class Integrator:
         self.rho_s_onaxis_t = torch.zeros(n_saves, self.Nt, **f32)
         self.I_onaxis_t = torch.zeros(n_saves, self.Nt, **f32)
 
-        self._rows = torch.zeros(7, self.Nr, dtype=torch.float64, device=dev)
+        self._rec_partial = torch.zeros(3, self.Nr, dtype=torch.float64, device=dev)
         self._r_w_dev = torch.from_numpy(self._r_w).to(dev)
         self._w0 = self._w1 = None
         self._utmp = None
@@ -299,39 +299,40 @@ class Integrator:
 
     # ------------------------------------------------------------------ recording
     def _record(self, z_now):
-        """integrator.py:179-224 with device-side reductions."""
-        k, g, cfg = self.k_save, self.g, self.cfg
-        u = self.u
-        t = self.t
+        """integrator.py:179-224 as one fp_record_history call: the inverse transform of
+        the spectral state, the reductions of loss_rates (operators.py:148-163) and every
+        history entry of record k are written on the device."""
+        k = self.k_save
+        if not 0 <= k < self.n_saves:
+            raise IndexError(f"record {k}: the histories hold {self.n_saves} records")
+        if self._mx is not None:
+            spec, c64, tw = self._mx.spec, 1, self._mx.m.tw
+        else:
+            spec, c64, tw = self._spec, 0, self.t.tw
         nb = 0 if self._bin_lo is None else int(self._bin_lo.numel())
-        call("fp_record_rows", ptr(u), ptr(self.rho), ptr(self.rho_s), ptr(self._rows),
-             ctypes.byref(self.nl.params(self.dz)), ctypes.byref(t.tables), self.dt, nb,
-             ptr(self._bin_lo), ptr(self._bin_hi), ptr(self._bins_tmp) if nb else None,
+        call("fp_record_history", ptr(spec), c64, ptr(tw), ptr(self.rho), ptr(self.rho_s),
+             ctypes.byref(self.nl.params(self.dz)), ctypes.byref(self.t.tables), self.dt, nb,
+             ptr(self._bin_lo), ptr(self._bin_hi), ctypes.byref(self._record_dst(k)),
              stream_ptr())
-        rows = self._rows
-        invE2, dt = g["invE2"], self.dt
-        self.fluence_rz[k] = (rows[0] * dt * invE2).float()
-        self.rho_rz[k] = rows[1].float()
-        self.rho_s_rz[k] = rows[2].float()
-        self._Imax_dev[k] = rows[3].max()
         self.z_saved[k] = float(z_now)
-        self.absorbed_rz[k] = rows[4].float()
-        if nb:
-            self.absorbed_rz_bins[k] = self._bins_tmp.float()
-        self.Ipeak_rz[k] = rows[3].float()
-        if self._it_snap is not None:
-            self.rho_rz_at[k] = self.rho[:, self._it_snap].float()
-        self._dEdz_dev[0, k] = (rows[5] * dt * self._r_w_dev).sum() * 1e4 * 1e6
-        self._dEdz_dev[1, k] = (rows[6] * dt * self._r_w_dev).sum() * 1e4 * 1e6
-        if self._t_stride > 0 and self.rho_rzt is not None:
-            rs, ts = self._r_stride, self._t_stride
-            self.rho_rzt[k] = self.rho[::rs, ::ts].float()
-            self.rho_s_rzt[k] = self.rho_s[::rs, ::ts].float()
-            self.I_rzt[k] = ((u[::rs, ::ts].abs() ** 2) * invE2).float()
-        self.rho_onaxis_t[k] = self.rho[0].float()
-        self.rho_s_onaxis_t[k] = self.rho_s[0].float()
-        self.I_onaxis_t[k] = ((u[0].abs() ** 2) * invE2).float()
         self.k_save += 1
+
+    def _record_dst(self, k):
+        """Device addresses of the entries at index k of every history."""
+        def at(x):
+            return None if x is None else x.data_ptr() + k * x.stride(0) * x.element_size()
+        cube = self._t_stride > 0 and self.rho_rzt is not None
+        dE = self._dEdz_dev.data_ptr()
+        return RecordDst(
+            fluence=at(self.fluence_rz), rho=at(self.rho_rz), rho_s=at(self.rho_s_rz),
+            ipeak=at(self.Ipeak_rz), absorbed=at(self.absorbed_rz), bins=at(self.absorbed_rz_bins),
+            rho_at=at(self.rho_rz_at), it_snap=self._it_snap or 0,
+            rho_onaxis=at(self.rho_onaxis_t), rho_s_onaxis=at(self.rho_s_onaxis_t),
+            I_onaxis=at(self.I_onaxis_t), r_stride=self._r_stride,
+            t_stride=self._t_stride if cube else 0, Nt_sub=self._Nt_sub if cube else 0,
+            rho_cube=at(self.rho_rzt), rho_s_cube=at(self.rho_s_rzt), I_cube=at(self.I_rzt),
+            r_w=self._r_w_dev.data_ptr(), partial=self._rec_partial.data_ptr(),
+            imax=at(self._Imax_dev), dedz_photo=dE + 8 * k, dedz_plasma=dE + 8 * (self.n_saves + k))
 
     def _cumulative_energy_fraction(self, dEdz_uJm):
         """Trapezoid cumulative integral of dE/dz over z, over U0 (integrator.py:226-236)."""

@@ -119,6 +119,69 @@ def test_graph_replay_matches_eager(precision):
     assert float(junk[0]) == 4.0
 
 
+@pytest.mark.parametrize("precision,Nt", [("fp64", 512), ("mixed", 512), ("fp64", 2048),
+                                          ("mixed", 2048)])
+def test_fused_record_matches_unfused(precision, Nt):
+    """fp_record_history (inverse transform, reductions and every history entry from the
+    spectral state, two launches) against the unfused record: inverse fp_fft_rows, then
+    fp_record_rows, then the per-history torch expressions of integrator.py:182-224.
+    Both sides are FP64 up to the float32 stores, so the histories agree to float32
+    rounding (the per-row sums run in another order when Nt < 2048)."""
+    import ctypes
+
+    import rapport_amd as R
+    from rapport_amd._lib import call, ptr, stream_ptr
+    _announce(f"fused record vs unfused, {precision} Nt={Nt}")
+    hot = dict(C1, Nt=Nt, nz=11, end=11 * 10e-6, energy_uJ=20.0)
+    kw = cfg_kwargs(hot, w0=30e-6, R_factor=20.0, save_stride=4, enable_ste=True, tau_ste=1e-12,
+                    rho_t_stride=7, rho_r_stride=3, absorb_time_bins_fs=(-100, -50, 0, 50, 100),
+                    rho_snapshot_t_fs=40.0, precision=precision)
+    it = R.Integrator(R.Config(**kw), envelope=R.envelope_lens_noise(**envelope_args(hot)))
+    it.propagate()
+    assert float(it.rho.max()) > 1e17
+
+    u = it.u
+    nb = int(it._bin_lo.numel())
+    rows = torch.zeros(7, it.Nr, dtype=torch.float64, device=u.device)
+    bins = torch.zeros(nb, it.Nr, dtype=torch.float64, device=u.device)
+    call("fp_record_rows", ptr(u), ptr(it.rho), ptr(it.rho_s), ptr(rows),
+         ctypes.byref(it.nl.params(it.dz)), ctypes.byref(it.t.tables), it.dt, nb,
+         ptr(it._bin_lo), ptr(it._bin_hi), ptr(bins), stream_ptr())
+    invE2, dt, rs, ts = it.g["invE2"], it.dt, it._r_stride, it._t_stride
+    I = (u.real ** 2 + u.imag ** 2) * invE2
+    expect = {
+        "fluence_rz": (rows[0] * dt * invE2).float(), "rho_rz": rows[1].float(),
+        "rho_s_rz": rows[2].float(), "Ipeak_rz": rows[3].float(), "absorbed_rz": rows[4].float(),
+        "absorbed_rz_bins": bins.float(), "rho_rz_at": it.rho[:, it._it_snap].float(),
+        "rho_rzt": it.rho[::rs, ::ts].float(), "rho_s_rzt": it.rho_s[::rs, ::ts].float(),
+        "I_rzt": I[::rs, ::ts].float(), "rho_onaxis_t": it.rho[0].float(),
+        "rho_s_onaxis_t": it.rho_s[0].float(), "I_onaxis_t": I[0].float(),
+    }
+    imax = float(rows[3].max())
+    dedz = [float((rows[q] * dt * it._r_w_dev).sum() * 1e4 * 1e6) for q in (5, 6)]
+
+    k = 1                                          # overwrite record 1 with the current state
+    others = {name: (getattr(it, name)[k - 1].clone(), getattr(it, name)[k + 1].clone())
+              for name in expect}
+    for name in expect:
+        getattr(it, name)[k].fill_(-1.0)
+    it.k_save = k
+    it._record(0.0)
+    torch.cuda.synchronize()
+    assert it.k_save == k + 1
+    for name, e in expect.items():
+        got = getattr(it, name)[k]
+        assert got.shape == e.shape, name
+        torch.testing.assert_close(got, e, rtol=1e-6, atol=0.0, msg=name)
+    assert float(it._Imax_dev[k]) == imax
+    np.testing.assert_allclose(float(it._dEdz_dev[0, k]), dedz[0], rtol=1e-12)
+    np.testing.assert_allclose(float(it._dEdz_dev[1, k]), dedz[1], rtol=1e-12)
+    assert dedz[0] > 0 and dedz[1] > 0
+    for name, (before, after) in others.items():   # the neighbouring records are untouched
+        assert torch.equal(getattr(it, name)[k - 1], before), name
+        assert torch.equal(getattr(it, name)[k + 1], after), name
+
+
 # params.json fields written by the reference (D3/sim/integrator.py:305-340)
 PARAMS_KEYS = {"n0", "n0_probe", "nc_probe_cm3", "lambda_probe_nm", "wavelength_nm", "n2", "U_g_eV",
                "Us_eV", "energy_uJ", "w0_um", "delta_t_fs", "code_fingerprint", "begin_um", "end_um",
