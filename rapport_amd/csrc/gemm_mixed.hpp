@@ -20,9 +20,23 @@
 // tiles). K slab 32, LDS double-buffered. The epilogue also reduces max|C| into a device
 // word for the next GEMM in the chain.
 #pragma once
+#include <type_traits>
+#include <utility>
+
 #include "common.hpp"
 
 namespace fpgemm16 {
+
+// f(std::integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time (register
+// arrays indexed by i stay in registers).
+template <typename F, int... I>
+FP_DEV void static_for_seq(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+FP_DEV void static_for(F&& f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -50,7 +64,7 @@ template <int BM, int BN, int WM = 2, int WN = 2> struct Smem {
 };
 
 // One BM x BN output tile (tm, tn) of C = alpha diag(rowscale) Y B (x phase).
-template <int BM, int BN, int WM = 2, int WN = 2>
+template <int BM, int BN, int WM = 2, int WN = 2, int NPF = 0>
 FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __restrict__ Yh,
                       const h8* __restrict__ Yl, int y_scale_exp, const float* __restrict__ B,
                       float* __restrict__ C, int M, int K, int Ncols, float alpha,
@@ -175,6 +189,39 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
     __builtin_amdgcn_s_setprio(0);
   };
 
+  // Phase prefetch (NPF > 0): every lane needs, for each of its outputs (m, n), the phase
+  // of the complex element (n & ~1, n | 1) of row m. It loads the one float at its own
+  // column n (re for even n, im for odd n) and takes the other from its neighbour lane
+  // (n ^ 1) by a shuffle in the epilogue. The loads of 16-row blocks a < NPF go out right
+  // after the operand loads of loop step pf0 + 2a, so that part of the phase table streams
+  // in while the last slabs are multiplied; the loads of the other blocks go out at the
+  // start of the epilogue (the staging registers are free by then) and land while the
+  // prefetched blocks are written. NPF = 0 reads each phase pair in the epilogue loop.
+  const char* Pc = reinterpret_cast<const char*>(phase);
+  const unsigned ncol4 = 4u * (unsigned)Ncols;
+  float php[FA][4][FB];   // unused (and removed) when NPF == 0
+  auto pload = [&](auto a_c) {
+    constexpr int a = decltype(a_c)::value;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const unsigned rowoff = (unsigned)(m0 + wm + a * 16 + 4 * lq + v) * ncol4;
+#pragma unroll
+      for (int b = 0; b < FB; ++b)
+        php[a][v][b] = *reinterpret_cast<const float*>(
+            Pc + rowoff + 4u * (unsigned)(n0 + wn + b * 16 + lr));
+    }
+  };
+  // in_loop: issue the blocks due at loop step s; else every block the loop did not issue
+  auto pload_at = [&](int s, bool in_loop) {
+    const int nsl = K / BK;
+    const int pf0 = nsl - 2 * NPF > 0 ? ((nsl - 2 * NPF) & ~1) : 0;
+    static_for<FA>([&](auto a_c) {
+      constexpr int a = decltype(a_c)::value;
+      const bool issued = a < NPF && pf0 + 2 * a < nsl;   // the loop visits even s < nsl
+      if (in_loop ? (issued && s == pf0 + 2 * a) : !issued) pload(a_c);
+    });
+  };
+
   // Slab s lives in LDS buffer s & 1. Invariant at the top of an even step s: slab s is
   // in buffer 0 (visible to all), R0 holds slab s + 1.
   const int nslab = K / BK;
@@ -184,6 +231,9 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
   __syncthreads();
   for (int s = 0; s < nslab; s += 2) {
     if (s + 2 < nslab) gload(s + 2, R1);
+    if constexpr (NPF > 0) {
+      if (phase) pload_at(s, true);
+    }
     mma(0);
     if (s + 1 >= nslab) break;
     sstore(1, R0);
@@ -196,11 +246,13 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
     }
   }
 
+  if constexpr (NPF > 0) {
+    if (phase) pload_at(nslab, false);
+  }
+
   // epilogue: for v_mfma_f32_16x16x32_f16 lane l, register v holds D[4 (l/16) + v][l % 16]
-  // (32-bit byte offsets from the base pointers, as for the loads; the phase of the
-  // complex element (re, im) = columns (n & ~1, n | 1) is one 8-byte load)
+  // (32-bit byte offsets from the base pointers, as for the loads)
   const float unscale = pow2f(-(y_scale_exp + sb_exp));
-  const char* Pc = reinterpret_cast<const char*>(phase);
   char* Cc = reinterpret_cast<char*>(C);
   float vmax = 0.0f;
 #pragma unroll
@@ -216,8 +268,18 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
         float val = acc[a][b][v] * rs;
         if (phase) {
           const float other = __shfl_xor(val, 1, 64);
-          const float2 ph = *reinterpret_cast<const float2*>(Pc + rowoff + 4u * (n & ~1u));
-          val = (n & 1u) ? (other * ph.y + val * ph.x) : (val * ph.x - other * ph.y);
+          float phx, phy;
+          if constexpr (NPF > 0) {
+            const float own_ph = php[a][v][b];
+            const float other_ph = __shfl_xor(own_ph, 1, 64);
+            phx = (n & 1u) ? other_ph : own_ph;
+            phy = (n & 1u) ? own_ph : other_ph;
+          } else {
+            const float2 ph = *reinterpret_cast<const float2*>(Pc + rowoff + 4u * (n & ~1u));
+            phx = ph.x;
+            phy = ph.y;
+          }
+          val = (n & 1u) ? (other * phy + val * phx) : (val * phx - other * phy);
         }
         vmax = fmaxf(vmax, fabsf(val));
         *reinterpret_cast<float*>(Cc + rowoff + 4u * n) = val;
@@ -246,7 +308,7 @@ FP_DEV int xcd_remap(int id, int nb) {
 // (one block per CU, up to 512 VGPR + AGPR per lane), the smaller ones two.
 template <int BM, int BN> struct TileOcc { static constexpr int value = (BM * BN >= 128 * 256) ? 1 : 2; };
 
-template <int BM, int BN, int WM = 2, int WN = 2>
+template <int BM, int BN, int WM = 2, int WN = 2, int NPF = 0>
 __global__ __launch_bounds__(WM * WN * 64, (WM * WN > 4 ? 1 : TileOcc<BM, BN>::value)) void hankel_gemm_f16x3(
     const h8* __restrict__ Yh, const h8* __restrict__ Yl, int y_scale_exp,
     const float* __restrict__ B, float* __restrict__ C, int M, int K, int Ncols, float alpha,
@@ -257,8 +319,8 @@ __global__ __launch_bounds__(WM * WN * 64, (WM * WN > 4 ? 1 : TileOcc<BM, BN>::v
   if (reset_slot && blockIdx.x == 0 && threadIdx.x == 0) *reset_slot = 0u;
   const int tilesM = M / BM, tilesN = Ncols / BN;
   const int lin = xcd_remap(blockIdx.x, tilesM * tilesN);
-  gemm_tile<BM, BN, WM, WN>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
-            rowscale, phase, bmax_in, cmax_out);
+  gemm_tile<BM, BN, WM, WN, NPF>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K,
+                                 Ncols, alpha, rowscale, phase, bmax_in, cmax_out);
 }
 
 }  // namespace fpgemm16

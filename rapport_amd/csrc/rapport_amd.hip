@@ -25,6 +25,12 @@
 
 #define FP_VERSION 100
 
+// 16-row blocks per wave whose phase values the single-instance mixed GEMM loads during
+// its last K slabs (gemm_mixed.hpp); the rest are loaded at the start of its epilogue.
+#ifndef FP_MIXED_GEMM_NPF
+#define FP_MIXED_GEMM_NPF 1
+#endif
+
 namespace {
 
 thread_local std::string g_err;
@@ -707,13 +713,15 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
   // tile of 4 waves at one block per CU measured slower: 85 vs 66 us.)
   const auto* yh = reinterpret_cast<const fpgemm16::h8*>(Yh);
   const auto* yl = reinterpret_cast<const fpgemm16::h8*>(Yl);
+  constexpr int NPF = FP_MIXED_GEMM_NPF;
   if (Nr % 256 == 0)
-    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<256, 128, 4, 2>),
+    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<256, 128, 4, 2, NPF>),
                        dim3((Nr / 256) * (Ncols / 128)), dim3(512), 0, S(stream), yh, yl,
                        y_scale_exp, B, C, Nr, Nr, Ncols, alpha, rowscale, phase, bmax_in, cmax_out,
                        reset_slot);
   else
-    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<128, 128>), dim3((Nr / 128) * (Ncols / 128)),
+    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<128, 128, 2, 2, NPF>),
+                       dim3((Nr / 128) * (Ncols / 128)),
                        dim3(256), 0, S(stream), yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols, alpha,
                        rowscale, phase, bmax_in, cmax_out, reset_slot);
   return check_launch("hankel_gemm_mixed");
