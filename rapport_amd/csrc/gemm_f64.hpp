@@ -88,9 +88,31 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64(const double* __restr
   sstore(0);
   __syncthreads();
   const int li = lane & 15, lk = lane >> 4;
+
+  // Phase values: every lane loads the double at its own output (m, n) -- re for even n,
+  // im for odd n -- and takes the other part of the complex phase from lane n ^ 1 by a
+  // shuffle. Row block a = 0 is loaded one K slab before the end, so it streams in under
+  // the last slab's MFMAs (the 512 tiles reach the epilogue together, with no work left to
+  // overlap a 64 MiB table read); in the epilogue block a + 1 is loaded before block a is
+  // written, so at most two blocks of phase values are live beside the accumulators.
+  double php[4][4][4];
+  auto pload = [&](int a) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + wm + a * 16 + lk + 4 * v;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int n = n0 + wn + b * 16 + li;
+        php[a][v][b] = (m < M && n < Ncols) ? phase[(size_t)m * Ncols + n] : 0.0;
+      }
+    }
+  };
+  const int pf_slab = nslab >= 2 ? nslab - 2 : -1;   // block 0 issued in that iteration
+
   for (int s = 0; s < nslab; ++s) {
     const int cur = s & 1;
     if (s + 1 < nslab) gload((s + 1) * BK);
+    if (phase && s == pf_slab) pload(0);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double av[4], bv[4];
@@ -110,9 +132,12 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64(const double* __restr
     }
   }
 
+  if (phase && pf_slab < 0) pload(0);
+
   // epilogue: for v_mfma_f64_16x16x4f64 lane l, register v holds D[l/16 + 4 v][l % 16]
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
+    if (phase && a + 1 < 4) pload(a + 1);   // lands while block a is written
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int m = m0 + wm + a * 16 + lk + 4 * v;
@@ -122,14 +147,12 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64(const double* __restr
         const int n = n0 + wn + b * 16 + li;
         double val = acc[a][b][v] * rs;
         if (phase) {
+          // even lane holds re, odd lane holds im of the same complex element (m, n >> 1)
           const double other = __shfl_xor(val, 1, 64);
-          const int f = n >> 1;
-          double pr = 0.0, pi = 0.0;
-          if (m < M && n < Ncols) {
-            pr = phase[((size_t)m * (Ncols >> 1) + f) * 2];
-            pi = phase[((size_t)m * (Ncols >> 1) + f) * 2 + 1];
-          }
-          // even lane holds re, odd lane holds im of the same (m, f)
+          const double own_ph = php[a][v][b];
+          const double other_ph = __shfl_xor(own_ph, 1, 64);
+          const double pr = (n & 1) ? other_ph : own_ph;
+          const double pi = (n & 1) ? own_ph : other_ph;
           val = (n & 1) ? (other * pi + val * pr) : (val * pr - other * pi);
         }
         if (m < M && n < Ncols) C[(size_t)m * Ncols + n] = val;
