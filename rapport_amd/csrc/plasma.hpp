@@ -154,10 +154,12 @@ FP_DEV St rate_step(St x, double Ia, double Wa, double Wsa, const fp_plasma_para
   return y;
 }
 
-// LDS needed by plasma_row (bytes): two NT-double rows + scan scratch.
-template <int NT, int TPB>
+// LDS needed by plasma_row (bytes): one NT-double row (two with STE: n_e and n_s) + scan
+// scratch. Without STE a 256-thread row then takes 37 KB, so four rows fit in one CU's LDS.
+template <int NT, int TPB, bool STE>
 struct PlasmaLds {
-  static constexpr int bytes = (2 * NT + 2 * (TPB + 1) + 6 * (TPB / 64)) * 8;
+  static constexpr int rows = STE ? 2 : 1;
+  static constexpr int bytes = (rows * NT + 2 * (TPB + 1) + 6 * (TPB / 64)) * 8;
 };
 
 // Solves one row. On entry sI[0..NT) holds I_k = |E_k|^2 * field_to_I in LDS
@@ -171,8 +173,8 @@ FP_DEV int plasma_row(double* smem, const fp_plasma_params& p, const double* __r
   constexpr int C = NT / TPB;
   constexpr int NW = TPB / 64;
   double* sI = smem;
-  double* sS = smem + NT;                 // second row buffer (n_s output)
-  double* sb0 = smem + 2 * NT;            // chunk start offsets (TPB+1)
+  double* sS = STE ? smem + NT : nullptr;                 // second row buffer (n_s output)
+  double* sb0 = smem + PlasmaLds<NT, TPB, STE>::rows * NT;  // chunk start offsets (TPB+1)
   double* sb1 = sb0 + (TPB + 1);
   double* swt = sb1 + (TPB + 1);          // wave totals, 6 per wave
   const int tid = threadIdx.x;

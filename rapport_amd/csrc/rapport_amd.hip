@@ -118,10 +118,22 @@ __global__ void absmax_kernel(const float* __restrict__ x, long long n, unsigned
 // ------------------------------------------------------------------ plasma
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-template <typename T, int NT, int TPB>
+template <typename T, int NT, int TPB, bool STE>
 struct PlasmaSmem {
-  static constexpr int bytes = cmax(fpplasma::PlasmaLds<NT, TPB>::bytes,
+  static constexpr int bytes = cmax(fpplasma::PlasmaLds<NT, TPB, STE>::bytes,
                                     fpfft::LdsSize<NT>::value * 2 * (int)sizeof(T));
+};
+
+// Row blocks per CU the carrier-scan kernel is built for (minimum, as launch bounds). A
+// complex64 row without STE takes 37 KB of LDS, so up to four 256-thread rows fit in one CU;
+// three keep the kernel at 164 VGPRs without spills (four would need 128 and spill). The
+// other variants take 64-70 KB of LDS or more than 256 VGPRs and keep one.
+#ifndef FP_PLASMA_OCC_C64
+#define FP_PLASMA_OCC_C64 3
+#endif
+template <typename T, bool STE>
+struct PlasmaOcc {
+  static constexpr int value = (!STE && sizeof(T) == 4) ? FP_PLASMA_OCC_C64 : 1;
 };
 
 // One row: (optional inverse FFT of a spectral row) -> intensity -> carrier scan.
@@ -168,7 +180,7 @@ FP_DEV void plasma_row_body(char* smem, size_t row, const T* __restrict__ src,
 }
 
 template <typename T, int NT, int TPB, bool STE, bool SPEC>
-__global__ __launch_bounds__(TPB) void plasma_kernel(const T* __restrict__ src,
+__global__ __launch_bounds__(TPB, (PlasmaOcc<T, STE>::value)) void plasma_kernel(const T* __restrict__ src,
                                                       double* __restrict__ rho,
                                                       double* __restrict__ rhos,
                                                       T* __restrict__ u_out,
@@ -177,7 +189,7 @@ __global__ __launch_bounds__(TPB) void plasma_kernel(const T* __restrict__ src,
                                                       const double* __restrict__ Ws,
                                                       const T* __restrict__ tw, int* status,
                                                       unsigned* clear_slots) {
-  __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<T, NT, TPB>::bytes];
+  __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<T, NT, TPB, STE>::bytes];
   if (clear_slots && blockIdx.x == 0 && threadIdx.x < 3) clear_slots[threadIdx.x] = 0u;
   plasma_row_body<T, NT, TPB, STE, SPEC>(smem, blockIdx.x, src, rho, rhos, u_out, p, Wg, Ws, tw,
                                          status);
@@ -247,7 +259,7 @@ FP_DEV int group_find(const BatchGroup& g, int item) {
 template <int NT, int TPB, bool STE>
 __global__ __launch_bounds__(TPB) void batch_plasma_kernel(const fp_batch_instance* __restrict__ insts,
                                                            BatchGroup g) {
-  __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<float, NT, TPB>::bytes];
+  __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<float, NT, TPB, STE>::bytes];
   const int total = g.begin[g.n];
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int j = group_find(g, item);
