@@ -329,7 +329,10 @@ void batch_gemm_kernel(const fp_batch_instance* __restrict__ insts, BatchGroup g
       __syncthreads();
       continue;
     }
-    fpgemm16::gemm_tile<BM, BN, WM, WN>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(I.Yh),
+    // phase prefetch of one 16-row block per wave where the registers allow it (the
+    // 128 x 128 and 128 x 256 tiles would spill)
+    constexpr int NPF = (BM == 128 && BN >= 128) ? 0 : 1;
+    fpgemm16::gemm_tile<BM, BN, WM, WN, NPF>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(I.Yh),
                                 reinterpret_cast<const fpgemm16::h8*>(I.Yl), I.y_scale_exp, B, C,
                                 I.Nr, I.Nr, 2 * I.Nt, alpha, rs, ph, bin, cout);
     __syncthreads();
