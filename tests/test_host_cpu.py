@@ -91,3 +91,54 @@ def test_keldysh_validate_product():
 def test_fingerprint_changes_with_sources(tmp_path):
     fp = R.code_fingerprint()
     assert re.fullmatch(r"[0-9a-f]{12}", fp)
+
+
+def _record_args(Nr=128, Nt=512, nbins=0, **dst):
+    """Arguments of fp_record_history with placeholder device addresses (never
+    dereferenced: every case below is rejected before a launch)."""
+    import ctypes
+    from rapport_amd import _lib
+    p = _lib.NLParams()
+    p.Nr, p.Nt = Nr, Nt
+    t = _lib.NLTables()
+    d = _lib.RecordDst()
+    for f in ("fluence", "rho", "rho_s", "ipeak", "absorbed", "rho_onaxis", "rho_s_onaxis",
+              "I_onaxis", "r_w", "partial", "imax", "dedz_photo", "dedz_plasma"):
+        setattr(d, f, 0x1000)
+    d.r_stride, d.t_stride, d.Nt_sub = 1, 0, 0
+    for k, v in dst.items():
+        setattr(d, k, v)
+    lohi = 0x1000 if nbins else None
+    return [0x1000, 0, 0x1000, 0x1000, None, ctypes.byref(p), ctypes.byref(t), 1e-16, nbins,
+            lohi, lohi, ctypes.byref(d), None]
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(fluence=None), "null destination"),
+    (dict(imax=None), "null destination"),
+    (dict(nbins=2), "bins"),
+    (dict(rho_at=0x1000, it_snap=512), "it_snap"),
+    (dict(rho_at=0x1000, it_snap=-1), "it_snap"),
+    (dict(t_stride=-1), "t_stride"),
+    (dict(t_stride=7, r_stride=0, Nt_sub=74), "r_stride"),
+    (dict(t_stride=7, r_stride=3, Nt_sub=73), "Nt_sub"),
+    (dict(t_stride=7, r_stride=3, Nt_sub=74), "null cube"),
+    (dict(Nt=300), "Nt must be"),
+    (dict(Nr=0), "Nr must be positive"),
+])
+def test_record_history_rejects_bad_arguments(kw, msg):
+    """fp_record_history validates its destinations and shapes on the host: the histories
+    are written at raw addresses, so a wrong stride or index must not reach a kernel."""
+    from rapport_amd import _lib
+    shape = {k: kw.pop(k) for k in ("Nr", "Nt", "nbins") if k in kw}
+    args = _record_args(**shape, **kw)
+    with pytest.raises(RuntimeError, match=msg):
+        _lib.call("fp_record_history", *args)
+
+
+def test_record_history_rejects_null_spec():
+    from rapport_amd import _lib
+    args = _record_args()
+    args[0] = None
+    with pytest.raises(RuntimeError, match="null pointer"):
+        _lib.call("fp_record_history", *args)
