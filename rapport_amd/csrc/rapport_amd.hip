@@ -124,10 +124,12 @@ struct PlasmaSmem {
                                     fpfft::LdsSize<NT>::value * 2 * (int)sizeof(T));
 };
 
-// Row blocks per CU the carrier-scan kernel is built for (minimum, as launch bounds). A
-// complex64 row without STE takes 37 KB of LDS, so up to four 256-thread rows fit in one CU;
-// three keep the kernel at 164 VGPRs without spills (four would need 128 and spill). The
-// other variants take 64-70 KB of LDS or more than 256 VGPRs and keep one.
+// Minimum waves per SIMD the carrier-scan kernel is built for (second launch-bounds
+// argument; it caps VGPRs at 512 / value). A 256-thread row is one wave per SIMD, so this is
+// also the number of row blocks per CU. A complex64 row without STE takes 37 KB of LDS, so up
+// to four rows fit in one CU; three keep the kernel at 164 VGPRs without spills (four would
+// need 128 and spill). The other variants take 64-70 KB of LDS or more than 256 VGPRs and
+// keep one.
 #ifndef FP_PLASMA_OCC_C64
 #define FP_PLASMA_OCC_C64 3
 #endif
