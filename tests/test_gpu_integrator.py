@@ -119,14 +119,15 @@ def test_graph_replay_matches_eager(precision):
     assert float(junk[0]) == 4.0
 
 
-@pytest.mark.parametrize("precision,Nt", [("fp64", 512), ("mixed", 512), ("fp64", 2048),
-                                          ("mixed", 2048)])
+@pytest.mark.parametrize("precision,Nt", [("fp64", 512), ("mixed", 512), ("fp64", 1024),
+                                          ("fp64", 2048), ("mixed", 2048), ("mixed", 4096)])
 def test_fused_record_matches_unfused(precision, Nt):
     """fp_record_history (inverse transform, reductions and every history entry from the
     spectral state, two launches) against the unfused record: inverse fp_fft_rows, then
     fp_record_rows, then the per-history torch expressions of integrator.py:182-224.
     Both sides are FP64 up to the float32 stores, so the histories agree to float32
-    rounding (the per-row sums run in another order when Nt < 2048)."""
+    rounding (the per-row sums run in another order when Nt < 2048). The sizes cover the four
+    block shapes of the row kernel (64, 128, 256 and 256 threads with 16 samples each)."""
     import ctypes
 
     import rapport_amd as R
