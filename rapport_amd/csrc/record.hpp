@@ -2,12 +2,12 @@
 // loss_rates, D3/sim/operators.py:148-163) in two launches, straight from the spectral
 // state into the float32 histories.
 //
-// record_history_kernel: one workgroup per radial row. It reads the row of the spectral
-// state (complex128, or the mixed path's complex64), forms the time-domain row with the
-// same inverse LDS transform and 1/NT scale as fft_rows_kernel (so the field matches the
-// unfused path bit for bit; the complex64 row is upcast after the scale), and reduces in
-// FP64: sum_t |u|^2, max_t rho, max_t rho_s, max_t I, sum_t dep, sum_t 2 photo I,
-// sum_t 2 plasma I, with I = |u|^2 invE2 and dep = 2 (photo + plasma) I dt 100. It writes
+// record_history_kernel: one workgroup per radial row, eight samples per thread. It reads
+// the row of the spectral state (complex128, or the mixed path's complex64), forms the
+// time-domain row with an inverse LDS transform and the 1/NT scale of fft_rows_kernel (the
+// complex64 row is upcast after the scale), and reduces in FP64: sum_t |u|^2, max_t rho,
+// max_t rho_s, max_t I, sum_t dep, sum_t 2 photo I, sum_t 2 plasma I, with
+// I = |u|^2 invE2 and dep = 2 (photo + plasma) I dt 100. It writes
 // this record's history entries as float32: fluence (sum |u|^2 dt invE2), rho, rho_s, peak
 // intensity, absorbed, the absorbed time bins, the density column at the snapshot time,
 // and, for row 0, the on-axis rows and, for rows on the radial stride, the subsampled

@@ -126,8 +126,9 @@ def test_fused_record_matches_unfused(precision, Nt):
     spectral state, two launches) against the unfused record: inverse fp_fft_rows, then
     fp_record_rows, then the per-history torch expressions of integrator.py:182-224.
     Both sides are FP64 up to the float32 stores, so the histories agree to float32
-    rounding (the per-row sums run in another order when Nt < 2048). The sizes cover the four
-    block shapes of the row kernel (64, 128, 256 and 256 threads with 16 samples each)."""
+    rounding (the per-row sums run in another order when Nt < 2048, and at Nt = 4096 the row
+    kernel's transform runs on 512 threads against 256 in fp_fft_rows). The sizes cover the
+    four block shapes of the row kernel (64, 128, 256 and 512 threads)."""
     import ctypes
 
     import rapport_amd as R
@@ -173,10 +174,18 @@ def test_fused_record_matches_unfused(precision, Nt):
     for name, e in expect.items():
         got = getattr(it, name)[k]
         assert got.shape == e.shape, name
-        torch.testing.assert_close(got, e, rtol=1e-6, atol=0.0, msg=name)
-    assert float(it._Imax_dev[k]) == imax
-    np.testing.assert_allclose(float(it._dEdz_dev[0, k]), dedz[0], rtol=1e-12)
-    np.testing.assert_allclose(float(it._dEdz_dev[1, k]), dedz[1], rtol=1e-12)
+        # pointwise samples: transform rounding is relative to the row's peak, so the
+        # absolute tolerance is 1e-6 of the array's maximum
+        torch.testing.assert_close(got, e, rtol=1e-6, atol=1e-6 * float(e.abs().max()), msg=name)
+    # the FP64 scalars: exact / summation-order close where the field is bitwise the unfused
+    # one; FP32 transform rounding where the row kernel's transform differs (mixed, 4096)
+    same_field = precision == "fp64" or Nt < 4096
+    if same_field:
+        assert float(it._Imax_dev[k]) == imax
+    np.testing.assert_allclose(float(it._Imax_dev[k]), imax, rtol=1e-6)
+    rt = 1e-12 if same_field else 1e-6
+    np.testing.assert_allclose(float(it._dEdz_dev[0, k]), dedz[0], rtol=rt)
+    np.testing.assert_allclose(float(it._dEdz_dev[1, k]), dedz[1], rtol=rt)
     assert dedz[0] > 0 and dedz[1] > 0
     for name, (before, after) in others.items():   # the neighbouring records are untouched
         assert torch.equal(getattr(it, name)[k - 1], before), name
