@@ -214,6 +214,97 @@ def test_split_kerr_only_is_spm():
     assert np.abs(rhs - ref).max() < 1e-12 * np.abs(ref).max()
 
 
+# split() one term at a time (D3/sim/operators.py:89-146). With self-steepening, space-time
+# focusing and the spectral filter off, T = U^-1 = 1; constant-modulus single-frequency
+# fields put every product in one FFT bin, so each spectral operator enters as a number.
+_NO_NL = dict(enable_kerr_instantaneous=False, enable_kerr_raman=False,
+              enable_photoionization_loss=False, enable_plasma_defocusing=False,
+              enable_plasma_absorption=False, enable_self_steepening=False,
+              enable_space_time_focusing=False, enable_spectral_filter=False)
+
+
+def _tone(g, m, I_Wcm2):
+    """a exp(2 pi i m n / Nt) on every row, |a|^2 invE2 = I: constant |u|, FFT bin m."""
+    Nr, Nt = g["rr"].shape
+    a = np.sqrt(I_Wcm2 / g["invE2"])
+    return np.broadcast_to(a * np.exp(2j * np.pi * m * np.arange(Nt) / Nt), (Nr, Nt)).copy()
+
+
+def _kappa(cfg):
+    return 3 * cfg.chi3 * cfg.omega0 ** 2 / (8 * cfg.komega * c ** 2)
+
+
+@pytest.mark.parametrize("kw", [dict(enable_photoionization_loss=True),
+                                dict(enable_plasma_absorption=True),
+                                dict(enable_photoionization_loss=True, enable_plasma_absorption=True)])
+def test_split_losses_cancel_alpha_without_spectral_operators(kw):
+    # the losses enter NL as -T FFT(photo u) and -plasma_pref rho u, and rhs adds alpha u:
+    # with T = U^-1 = 1 they cancel exactly (a sign error would leave 2 alpha u)
+    cfg = _small_cfg(**{**_NO_NL, **kw})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g, scale=30.0)
+    rho = np.random.default_rng(4).uniform(0, 1e21, u.shape)
+    rhs, a = nl.split(u, rho, None)
+    assert a.max() > 0
+    assert np.abs(rhs).max() <= 1e-12 * np.abs(a * u).max()
+
+
+def test_split_plasma_defocusing_is_the_drude_index():
+    # defocusing alone: rhs = -i omega0 / (2 n0 c rho_c) * x / (1 + x) * rho u, x = (omega0 tau_c)^2,
+    # rho_c = eps0 m omega0^2 / e^2 (the Drude plasma index; rho in m^-3 = 1e6 x cm^-3)
+    cfg = _small_cfg(**{**_NO_NL, "enable_plasma_defocusing": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g, scale=10.0)
+    rho = np.random.default_rng(5).uniform(0, 1e20, u.shape)
+    rhs, a = nl.split(u, rho, None)
+    w0 = 2 * np.pi * c / cfg.wavelength
+    rho_c = epsilon_0 * cfg.meff_drude_rel * m_e * w0 ** 2 / q_e ** 2
+    x = (w0 * cfg.tau_c) ** 2
+    ref = -1j * w0 / (2 * cfg.n0 * c * rho_c) * x / (1 + x) * (rho * 1e6) * u
+    assert np.all(a == 0)
+    assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("m", [3, -5])
+@pytest.mark.parametrize("stf", [False, True])
+def test_split_single_frequency_spectral_operators(m, stf):
+    # Kerr + photo-ionisation on one tone f_m = m / (Nt dt): the Kerr term carries T^2, the
+    # photo term T, both U^-1, and alpha u neither: rhs = [U^-1 (i kappa (1-f_R) |a|^2 T^2
+    # - T photo) + photo] u with T = 1 + f_m / f0 and U = 1 + k1 2 pi f_m / k
+    cfg = _small_cfg(**{**_NO_NL, "enable_kerr_instantaneous": True,
+                        "enable_photoionization_loss": True, "enable_self_steepening": True,
+                        "enable_space_time_focusing": stf})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _tone(g, m, 4e13)
+    rhs, a = nl.split(u, np.zeros(u.shape), None)
+    photo = a[0, 0]
+    assert photo > 0 and np.allclose(a, photo, rtol=1e-12, atol=0)
+    f_m = m / (cfg.Nt * g["dt"])
+    T = 1 + f_m / (c / cfg.wavelength)
+    Uinv = 1 / max(1 + g["k1"] * 2 * np.pi * f_m / cfg.komega, 0.1) if stf else 1.0
+    a2 = np.abs(u[0, 0]) ** 2
+    ref = (Uinv * (1j * _kappa(cfg) * (1 - cfg.f_R) * a2 * T ** 2 - T * photo) + photo) * u
+    assert abs(T - 1) > 1e-3 and (not stf or abs(Uinv - 1) > 1e-4)
+    assert np.abs(rhs - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_split_raman_of_constant_intensity():
+    # Raman alone on a constant |u|^2 = A: the response integral is the DC value of R_f
+    # (its closed form is pinned above), rhs = i kappa f_R A Re(R_f[0]) u
+    cfg = _small_cfg(**{**_NO_NL, "enable_kerr_raman": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _tone(g, 0, 3e13)
+    rhs, _ = nl.split(u, np.zeros(u.shape), None)
+    A = np.abs(u[0, 0]) ** 2
+    ref = 1j * _kappa(cfg) * cfg.f_R * A * g["R_f"][0].real * u
+    assert 0.9 < g["R_f"][0].real < 1.1          # unit-area response, sampled
+    assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
