@@ -305,6 +305,20 @@ def test_split_raman_of_constant_intensity():
     assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+def test_split_ste_index_term_alone():
+    # the STE index change alone: extra = +i ste_pref rho_s u, so rhs = i ste_pref rho_s u
+    # (ste_pref itself is pinned by the recorded delta n_STE above)
+    cfg = _small_cfg(**{**_NO_NL, "enable_ste": True, "enable_ste_index": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g, scale=10.0)
+    rho_s = np.random.default_rng(6).uniform(0, 1e20, u.shape)
+    rhs, a = nl.split(u, np.zeros(u.shape), rho_s)
+    assert g["ste_pref"] > 0 and np.all(a == 0)
+    ref = 1j * g["ste_pref"] * rho_s * u
+    assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
