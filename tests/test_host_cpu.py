@@ -142,3 +142,26 @@ def test_record_history_rejects_null_spec():
     args[0] = None
     with pytest.raises(RuntimeError, match="null pointer"):
         _lib.call("fp_record_history", *args)
+
+
+def test_header_is_a_c_abi(tmp_path):
+    """include/rapport_amd.h compiles as C99, and a C program links the library through it
+    and calls an entry point (fp_version needs no GPU)."""
+    import shutil
+    from rapport_amd import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "abi.c"
+    src.write_text('#include "rapport_amd.h"\n#include <stdio.h>\n'
+                   "int main(void) {\n"
+                   "  fp_record_dst d = {0}; fp_nl_params p = {0}; (void)d; (void)p;\n"
+                   '  printf("%d\\n", fp_version());\n'
+                   "  return 0;\n}\n")
+    exe = tmp_path / "abi"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-L", libdir, "-lrapport_amd", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True,
+                         env={**os.environ, "LD_LIBRARY_PATH": libdir})
+    assert int(out.stdout) >= 100
