@@ -45,24 +45,29 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64(const double* __restr
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
 
-  // global->LDS staging: each thread moves 8 consecutive doubles of A (row ldk
-  // of the Y^T slab) and 8 of B per slab; full tiles use 16-byte loads/stores.
+  // global->LDS staging: each thread moves 4 double pairs of A (row ldk of the Y^T slab)
+  // and 4 of B per slab, pair i at column ldc + 32 i: the 16 threads of a row cover 256
+  // contiguous bytes per pair, so both the global loads and the 16-byte LDS stores are
+  // contiguous across lanes (no LDS bank conflicts); full tiles use 16-byte accesses.
   const int ldk = tid >> 4;
-  const int ldc = (tid & 15) * 8;
+  const int ldc = (tid & 15) * 2;
   const bool full = (m0 + BM <= M) && (n0 + BN <= Ncols) && (K % BK == 0);
   double2 ra[4], rb[4];
 
   auto gload = [&](int k0) {
     const int k = k0 + ldk;
     if (full) {
-      const double2* pa = reinterpret_cast<const double2*>(Yt + (size_t)k * M + m0 + ldc);
-      const double2* pb = reinterpret_cast<const double2*>(B + (size_t)k * Ncols + n0 + ldc);
+      const double* pa = Yt + (size_t)k * M + m0 + ldc;
+      const double* pb = B + (size_t)k * Ncols + n0 + ldc;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) { ra[i] = pa[i]; rb[i] = pb[i]; }
+      for (int i = 0; i < 4; ++i) {
+        ra[i] = *reinterpret_cast<const double2*>(pa + 32 * i);
+        rb[i] = *reinterpret_cast<const double2*>(pb + 32 * i);
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int m = m0 + ldc + 2 * i, n = n0 + ldc + 2 * i;
+        const int m = m0 + ldc + 32 * i, n = n0 + ldc + 32 * i;
         ra[i].x = (k < K && m < M) ? Yt[(size_t)k * M + m] : 0.0;
         ra[i].y = (k < K && m + 1 < M) ? Yt[(size_t)k * M + m + 1] : 0.0;
         rb[i].x = (k < K && n < Ncols) ? B[(size_t)k * Ncols + n] : 0.0;
@@ -71,10 +76,11 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64(const double* __restr
     }
   };
   auto sstore = [&](int buf) {
-    double2* sa = reinterpret_cast<double2*>(&As[buf][ldk][ldc]);
-    double2* sb = reinterpret_cast<double2*>(&Bs[buf][ldk][ldc]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) { sa[i] = ra[i]; sb[i] = rb[i]; }
+    for (int i = 0; i < 4; ++i) {
+      *reinterpret_cast<double2*>(&As[buf][ldk][ldc + 32 * i]) = ra[i];
+      *reinterpret_cast<double2*>(&Bs[buf][ldk][ldc + 32 * i]) = rb[i];
+    }
   };
 
   d4 acc[4][4];
