@@ -319,6 +319,35 @@ def test_split_ste_index_term_alone():
     assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+def test_split_zero_field_samples_have_zero_rhs():
+    # rhs is set to 0 where |u|^2 < 1e-30 (operators.py:145): with self-steepening the
+    # spectral terms spread into the field's gaps, so without the mask rhs would not vanish there
+    cfg = _small_cfg(**{**_NO_NL, "enable_kerr_instantaneous": True,
+                        "enable_self_steepening": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g, scale=10.0)
+    u[:, : cfg.Nt // 4] = 0.0
+    rhs, _ = nl.split(u, np.zeros(u.shape), None)
+    assert np.all(rhs[:, : cfg.Nt // 4] == 0)
+    lit = np.abs(u[:, cfg.Nt // 2]) ** 2 > 1e-20          # rows the beam reaches
+    assert lit.sum() > 10 and np.abs(rhs[lit, cfg.Nt // 2]).min() > 0
+
+
+def test_split_alpha_is_clipped_at_zero():
+    # alpha = max(nan_to_num(photo + plasma_pref rho), 0) (operators.py:98-107): a negative
+    # density (outside the physical range) gives alpha = 0, not a gain
+    cfg = _small_cfg(**{**_NO_NL, "enable_plasma_absorption": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g)
+    rho = np.full(u.shape, -1e20)
+    rho[0] = 1e20
+    _, a = nl.split(u, rho, None)
+    assert np.all(a[1:] == 0)
+    assert np.allclose(a[0], g["sigmaomega"] / 2 * 100 * 1e20, rtol=1e-14, atol=0)
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
