@@ -464,6 +464,23 @@ def test_linear_only_run_follows_gaussian_beam_and_z_labels():
         assert np.abs(got - ref).max() < 2e-6 * ref.max()
 
 
+def test_step_ends_with_the_radial_absorber():
+    # Integrator.step multiplies the stepped field by mask_r = exp(-(r / 0.9 R)^20)
+    # (integrator.py:148, grids.py:138). A flat field reaching the grid edge and a 1 nm step
+    # (linear-only: the propagation is the identity to ~1e-5) must come back as u0 mask_r.
+    cfg = _small_cfg(nz=1, begin=0.0, end=1e-9, **{**_NO_NL})
+    flat = lambda rr, tt, cfg_, g_: np.full(rr.shape, 1e8 + 0j)   # noqa: E731
+    it = O.Integrator(cfg, envelope=flat)
+    u0 = np.array(it.u, copy=True)
+    it.step(it.dz)
+    r = it.g["rlist"]
+    R = it.g["R"]
+    mask = np.exp(-(r / (0.9 * R)) ** 20)
+    assert mask[-1] < 0.2 and mask[0] == 1.0        # the absorber matters at the edge
+    ref = u0 * mask[:, None]
+    assert np.abs(it.u - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
 def test_record_invariants():
     cfg = _small_cfg(nz=2, end=20e-6, N=65, Nt=128, energy_uJ=40.0, w0=40e-6,
                      absorb_time_bins_fs=(-400, -50, 0, 50, 400),
