@@ -481,6 +481,22 @@ def test_step_ends_with_the_radial_absorber():
     assert np.abs(it.u - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
+def test_step_alpha_half_steps_total_one_absorption_length():
+    # step() applies exp(-alpha dz / 2) before and after the RK4 stage (integrator.py:141-147).
+    # With plasma absorption alone on a uniform density (T = U^-1 = 1, so the RK4 rhs is 0 and
+    # alpha commutes with the linear steps) a step is the linear-only step times exp(-alpha dz)
+    kw = dict(nz=1, begin=0.0, end=5e-6)
+    it_a = O.Integrator(_small_cfg(**kw, **{**_NO_NL, "enable_plasma_absorption": True}))
+    it_l = O.Integrator(_small_cfg(**kw, **_NO_NL))
+    for it in (it_a, it_l):
+        it.rho[:] = 1e20
+        it.step(it.dz)
+    alpha = it_a.g["sigmaomega"] / 2 * 100 * 1e20
+    assert alpha * it_a.dz > 1e-3                   # a visible attenuation
+    ref = it_l.u * np.exp(-alpha * it_a.dz)
+    assert np.abs(it_a.u - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_record_invariants():
     cfg = _small_cfg(nz=2, end=20e-6, N=65, Nt=128, energy_uJ=40.0, w0=40e-6,
                      absorb_time_bins_fs=(-400, -50, 0, 50, 400),
