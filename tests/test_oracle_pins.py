@@ -497,6 +497,40 @@ def test_step_alpha_half_steps_total_one_absorption_length():
     assert np.abs(it_a.u - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+def test_results_mirror_the_radial_axis():
+    # _results mirrors every (z, r) map as hstack([a[:, ::-1], a]) and r as [-r[::-1], r]
+    # (integrator.py:238-298): symmetric about the centre, the axis odd
+    cfg = _small_cfg(nz=2, end=20e-6, N=33, Nt=64, rho_t_stride=0)
+    res = O.Integrator(cfg).propagate()
+    Nr = cfg.N - 1
+    assert np.array_equal(res["r"], np.concatenate([-res["rlist"][::-1], res["rlist"]]))
+    for k in ("fluence_rz", "rho_rz", "Ipeak_rz", "absorbed_rz"):
+        a = res[k]
+        assert a.shape[1] == 2 * Nr, k
+        assert np.array_equal(a[:, :Nr], a[:, Nr:][:, ::-1]), k
+        assert np.abs(a[:, Nr]).max() >= np.abs(a[:, -1]).max(), k   # centre outward
+
+
+def test_lut_rate_index_and_interpolation():
+    # interpolate_pi_rate (kernels.py:14-26): fidx = (log10 I - log_min) inv_step, truncated;
+    # at the table's own sample points it returns the table, and at i + 3/4 it interpolates
+    # from sample i (rounding would interpolate from i + 1, off by the table's curvature)
+    cfg = _small_cfg()
+    g = O.build_grids(cfg)
+    k = g["keldysh"]
+    tab, n = k["W_LUT"], k["NLUT"]
+    lmin, inv = k["logImin"], k["inv_dlog"]
+    i = np.array([3, 100, 2047, n - 3])
+    I_at = 10.0 ** (lmin + (i + 1e-9) / inv)          # just above sample i
+    I_mid = 10.0 ** (lmin + (i + 0.75) / inv)
+    args = (n, lmin, inv, k["Imin"], k["Imax"])
+    assert np.allclose(O.lut_rate(I_at, tab, *args), tab[i], rtol=1e-6, atol=0)
+    assert np.allclose(O.lut_rate(I_mid, tab, *args), tab[i] + 0.75 * (tab[i + 1] - tab[i]),
+                       rtol=1e-9, atol=0)
+    # the table spans the PCHIP knots' decades uniformly in log10 I (4096 points, 4095 steps)
+    assert n == 4096 and abs(inv * (np.log10(9.99e16) - np.log10(1.0)) - 4095) < 1e-9
+
+
 def test_record_invariants():
     cfg = _small_cfg(nz=2, end=20e-6, N=65, Nt=128, energy_uJ=40.0, w0=40e-6,
                      absorb_time_bins_fs=(-400, -50, 0, 50, 400),
