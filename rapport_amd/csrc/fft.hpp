@@ -104,8 +104,12 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
           if (r + 1 < R) w = w * w1;
         }
       } else {
-        // FP32: table loads of w^1, w^4 (and w^8 for R = 16); the rest by at most
-        // two products, keeping every twiddle within a few ulp.
+        // FP32: one table load, w^1, per butterfly; the other powers by products
+        // (w^2 = w w, w^4 = w^2 w^2, w^8 = w^4 w^4, then one product each): at most
+        // 7 x 2^-24 from exp(-2 pi i m / N) for R = 8 and 15 x 2^-24 for R = 16
+        // (N = 4096). The table loads are scattered (lane stride STRIDE) and cost
+        // more than the products: loading w^4 (and w^8) as well is 2.7 % slower per
+        // mixed z-step at 1024 x 4096, loading all R - 1 powers 25 % slower.
         cpx<T> w[R];
         w[0] = mk<T>(T(1), T(0));
         w[1] = tw[k * STRIDE];
@@ -114,13 +118,13 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
           w[3] = w[2] * w[1];
         }
         if (R >= 8) {
-          w[4] = tw[4 * k * STRIDE];
+          w[4] = w[2] * w[2];
           w[5] = w[4] * w[1];
           w[6] = w[4] * w[2];
           w[7] = w[4] * w[3];
         }
         if (R >= 16) {
-          w[8] = tw[8 * k * STRIDE];
+          w[8] = w[4] * w[4];
 #pragma unroll
           for (int r = 9; r < 16; ++r) w[r] = w[8] * w[r - 8];
         }
