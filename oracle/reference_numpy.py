@@ -21,6 +21,10 @@ marked EXTENSION and default to the reference behaviour:
   seeded complex noise (the workload inputs).
 
 Parity status per function (what pins it) is listed in ``PINS`` at the bottom.
+Parity partly unpinned: ``NonlinearOperator.split``'s individual terms on a
+general (multi-frequency) field, pinned only on special fields and through the
+additivity of the terms; and the SciPy-vs-cupy library arithmetic (PCHIP,
+j0/j1, FFT), pinned only through closed forms.
 """
 
 from __future__ import annotations
@@ -761,10 +765,18 @@ PINS = {
     "build_grids.ste_pref": "Delta n_STE(1e20)=3.1131e-3 (reference record)",
     "LinearOperator": "Gaussian beam q-parameter diffraction, chirped-pulse dispersion, energy",
     "NonlinearOperator.split": "toggles-off => rhs=alpha=0; Kerr-only => i kappa |u|^2 u; "
-                               "photo/alpha pointwise closed forms; T-hat off => T=1",
+                               "photo/alpha pointwise closed forms; T-hat off => T=1; "
+                               "each term alone on single tones / flat fields (losses cancel "
+                               "alpha u, Drude index, Raman Re R_f[0], STE index, T^2 / T / "
+                               "1/U placement); zero-field mask; alpha clip; on a general "
+                               "field the all-terms rhs and alpha are the sums of the "
+                               "one-term runs",
     "rate_equation_rows": "constant-I exact solution of dx/dt=S+Lx; independent scalar "
-                          "restatement; non-negativity and rho<=rho_max",
+                          "restatement; non-negativity and rho<=rho_max; lut_rate at its "
+                          "knots and at i + 3/4 (truncating index, 4095 log10 steps)",
     "Integrator": "linear-only run = linear Gaussian propagation; nz+1 steps, z labels; "
                   "record invariants (bins sum, rho_at<=rho_rz, max Ipeak = max Imax); "
-                  "result.npz size estimate 0.99 GB / 86% cube at the 1030 nm config",
+                  "result.npz size estimate 0.99 GB / 86% cube at the 1030 nm config; "
+                  "radial absorber mask_r; two alpha half-steps = one absorption length; "
+                  "outputs mirrored about r = 0",
 }

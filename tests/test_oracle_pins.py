@@ -348,6 +348,45 @@ def test_split_alpha_is_clipped_at_zero():
     assert np.allclose(a[0], g["sigmaomega"] / 2 * 100 * 1e20, rtol=1e-14, atol=0)
 
 
+def test_split_terms_add_across_toggles():
+    # split() sums independent terms (operators.py:89-146): the Kerr and Raman parts of kerr_I,
+    # the photo term, the plasma term (coefficient [absorb] + [defoc] (phase - 1)), the STE
+    # index term, and alpha u with alpha = photo + [absorb] plasma_pref rho (>= 0, so the
+    # clip is inert). With self-steepening, space-time focusing and the spectral filter on,
+    # T and 1/U mix the frequencies of a general field, and still the rhs with every term on
+    # is the sum of the rhs with one term on. This pins how the terms combine, which the
+    # one-term pins above leave open (a term that overwrites instead of adding fails).
+    ops = dict(enable_self_steepening=True, enable_space_time_focusing=True,
+               enable_spectral_filter=True)
+    terms = [dict(enable_kerr_instantaneous=True), dict(enable_kerr_raman=True),
+             dict(enable_photoionization_loss=True), dict(enable_plasma_absorption=True),
+             dict(enable_plasma_defocusing=True), dict(enable_ste=True, enable_ste_index=True)]
+    every = {k: v for t in terms for k, v in t.items()}
+    cfg = _small_cfg(**{**_NO_NL, **ops, **every})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    u = _field(cfg, g, seed=4, scale=40.0)           # peak 2.5e14 W/cm^2: photo term alive
+    rng = np.random.default_rng(8)
+    rho = rng.uniform(0, 1e20, u.shape)
+    rho_s = rng.uniform(0, 1e20, u.shape)
+    rhs_all, a_all = nl.split(u, rho, rho_s)
+    parts, alphas = [], []
+    for t in terms:
+        c1 = _small_cfg(**{**_NO_NL, **ops, **t})
+        g1 = O.build_grids(c1)
+        for key in ("T_op", "inv_U_nl", "R_f", "sigmaomega", "komega"):
+            assert np.array_equal(g1[key], g[key]), key
+        r1, a1 = O.NonlinearOperator(c1, g1).split(u, rho, rho_s)
+        parts.append(r1)
+        alphas.append(a1)
+    scale = sum(np.abs(r).max() for r in parts)
+    for t, r in zip(terms, parts):                   # every term matters at this tolerance
+        assert np.abs(r).max() > 1e-6 * scale, t
+    assert np.abs(rhs_all - sum(parts)).max() <= 1e-12 * scale
+    assert np.abs(a_all - sum(alphas)).max() <= 1e-12 * np.abs(a_all).max()
+    assert np.abs(a_all).max() > 0 and np.abs(alphas[2]).max() > 0 and np.abs(alphas[3]).max() > 0
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
