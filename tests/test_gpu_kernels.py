@@ -62,6 +62,39 @@ def test_fft_rows(Nt):
     assert err < 1e-13, err
 
 
+@pytest.mark.parametrize("Nt", [512, 1024, 2048, 4096])
+def test_fft_rows_c64(Nt):
+    """complex64 row transform (radix-8 passes, radix 16 at Nt = 4096) against NumPy's FP64
+    transform of the same complex64 rows. The twiddles other than w come from products
+    (w^2 = w w, w^4 = w^2 w^2, ...), within 15 x 2^-24 of exp(-2 pi i m / Nt); the bounds
+    below are about 2.5x the measured FP32 rounding. A wrong or conjugated twiddle, a swapped
+    output position or a missing 1/Nt gives errors of order 1."""
+    from rapport_amd import _lib
+    _announce(f"fft_rows_c64 Nt={Nt}")
+    rng = np.random.default_rng(Nt + 1)
+    Nr = 64
+    x = (rng.standard_normal((Nr, Nt)) + 1j * rng.standard_normal((Nr, Nt))).astype(np.complex64)
+    tw = _cuda(np.exp(-2j * np.pi * np.arange(Nt) / Nt).astype(np.complex64))
+    xd = _cuda(x)
+    yd = torch.empty_like(xd)
+    _lib.call("fp_fft_rows_c64", _lib.ptr(xd), _lib.ptr(yd), Nr, Nt, 0, _lib.ptr(tw),
+              _lib.stream_ptr())
+    ref = np.fft.fft(x.astype(np.complex128), axis=1)
+    got = yd.cpu().numpy().astype(np.complex128)
+    rms = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    mx = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"fft_rows_c64 Nt={Nt} forward rel-rms {rms:.2e} rel-max {mx:.2e}", flush=True)
+    assert rms < 6e-7 and mx < 1.5e-6, (rms, mx)     # measured 1.7-3.0e-7 / 2.4-4.7e-7
+    _lib.call("fp_fft_rows_c64", _lib.ptr(yd), _lib.ptr(xd), Nr, Nt, 1, _lib.ptr(tw),
+              _lib.stream_ptr())
+    back = xd.cpu().numpy().astype(np.complex128)
+    x128 = x.astype(np.complex128)
+    rms = np.linalg.norm(back - x128) / np.linalg.norm(x128)
+    mx = np.abs(back - x128).max() / np.abs(x128).max()
+    print(f"fft_rows_c64 Nt={Nt} round trip rel-rms {rms:.2e} rel-max {mx:.2e}", flush=True)
+    assert rms < 1e-6 and mx < 1.5e-6, (rms, mx)     # measured 2.4-4.3e-7 / 3.1-5.9e-7
+
+
 @pytest.mark.parametrize("Nr,Nt", [(128, 512), (200, 512), (1024, 4096)])
 def test_hankel_gemm(Nr, Nt):
     from rapport_amd import _lib
