@@ -23,8 +23,9 @@ marked EXTENSION and default to the reference behaviour:
 Parity status per function (what pins it) is listed in ``PINS`` at the bottom.
 Parity partly unpinned: ``NonlinearOperator.split``'s individual terms on a
 general (multi-frequency) field, pinned only on special fields and through the
-additivity of the terms; and the SciPy-vs-cupy library arithmetic (PCHIP,
-j0/j1, FFT), pinned only through closed forms.
+additivity of the terms; and the SciPy-vs-cupy library arithmetic for j0/j1
+and the FFT, pinned only through closed forms (PCHIP is pinned to its published
+Fritsch-Carlson slopes).
 """
 
 from __future__ import annotations

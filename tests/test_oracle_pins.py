@@ -387,6 +387,57 @@ def test_split_terms_add_across_toggles():
     assert np.abs(a_all).max() > 0 and np.abs(alphas[2]).max() > 0 and np.abs(alphas[3]).max() > 0
 
 
+def _pchip_slopes(x, y):
+    """Knot derivatives of the monotone piecewise-cubic Hermite interpolant as published
+    (Fritsch & Carlson 1980, in the form of Moler, Numerical Computing with MATLAB, 3.4):
+    0 where the secants change sign or vanish, else the weighted harmonic mean of the two
+    secants with weights 2 h_k + h_(k-1) and h_k + 2 h_(k-1); at each end the one-sided
+    three-point formula, set to 0 if it opposes the first secant and limited to 3 times
+    that secant where the first two secants differ in sign."""
+    h = np.diff(x)
+    m = np.diff(y) / h
+    d = np.zeros_like(y)
+    for k in range(1, len(x) - 1):
+        if m[k - 1] * m[k] > 0:
+            w1, w2 = 2 * h[k] + h[k - 1], h[k] + 2 * h[k - 1]
+            d[k] = (w1 + w2) / (w1 / m[k - 1] + w2 / m[k])
+
+    def end(h0, h1, m0, m1):
+        e = ((2 * h0 + h1) * m0 - h0 * m1) / (h0 + h1)
+        if np.sign(e) != np.sign(m0):
+            return 0.0
+        if np.sign(m0) != np.sign(m1) and abs(e) > abs(3 * m0):
+            return 3 * m0
+        return e
+    d[0] = end(h[0], h[1], m[0], m[1])
+    d[-1] = end(h[-1], h[-2], m[-1], m[-2])
+    return d
+
+
+def test_pchip_is_the_published_monotone_interpolant():
+    # the photo-ionisation rate W(I) is a PCHIP through the Keldysh knots (keldysh.py:144-150);
+    # the oracle and the host tables use SciPy's PchipInterpolator where the reference uses
+    # cupyx's. Both follow the published Fritsch-Carlson slopes: pin SciPy's knot derivatives
+    # to that rule on the real knots (both channels) and on a small set with a sign change,
+    # a flat run and an overshooting end.
+    g = O.build_grids(_small_cfg())
+    k = g["keldysh"]
+    for W in (k["W_knots_g"], k["W_knots_s"]):
+        x = k["f_spline"].x
+        sp = O.PchipInterpolator(x, W)
+        ref = _pchip_slopes(x, W)
+        got = np.append(sp.c[2], sp.derivative()(x[-1]))
+        assert np.allclose(got, ref, rtol=1e-10, atol=1e-300)
+        assert np.allclose(sp(x), W, rtol=1e-12, atol=0)
+    x = np.array([0.0, 1.0, 1.5, 3.0, 4.0, 4.2, 6.0])
+    y = np.array([0.0, 2.0, 2.0, 1.0, 3.5, 3.6, 3.0])
+    sp = O.PchipInterpolator(x, y)
+    ref = _pchip_slopes(x, y)
+    assert ref[1] == 0 and ref[2] == 0 and ref[3] == 0   # flat run, then a sign change
+    got = np.append(sp.c[2], sp.derivative()(x[-1]))
+    assert np.allclose(got, ref, rtol=1e-13, atol=1e-15)
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
