@@ -438,6 +438,38 @@ def test_pchip_is_the_published_monotone_interpolant():
     assert np.allclose(got, ref, rtol=1e-13, atol=1e-15)
 
 
+def _bessel_series(n, x, digits=80):
+    """J_n(x) = sum_m (-1)^m (x/2)^(2m+n) / (m! (m+n)!), summed in `digits`-digit decimal
+    arithmetic (no cancellation loss for the arguments used here)."""
+    from decimal import Decimal, localcontext
+    with localcontext() as ctx:
+        ctx.prec = digits
+        h = Decimal(repr(float(x))) / 2
+        term = h ** n
+        for k in range(1, n + 1):
+            term /= k
+        total, m = term, 0
+        while abs(term) > Decimal(10) ** (-digits + 10) or m < 5:
+            m += 1
+            term = -term * h * h / (m * (m + n))
+            total += term
+        return float(total)
+
+
+def test_bessel_functions_are_the_power_series():
+    # the Hankel matrix Y (grids.py) is built from j0 / j1 and the zeros of J0; the oracle
+    # and the host use SciPy's, the reference cupyx's. Pin SciPy's values to the defining
+    # series: the first 20 zeros of J0 (arguments up to 62), and j0 / j1 at spread points.
+    z = O.jn_zeros(0, 20)
+    for zk in z:
+        assert abs(_bessel_series(0, zk)) < 1e-14
+        assert abs(O.j1(zk) - _bessel_series(1, zk)) < 1e-14
+    assert np.all(np.diff(z) > 3.0) and abs(z[0] - 2.404825557695773) < 1e-15
+    for x in np.linspace(0.05, 60.0, 37):
+        assert abs(O.j0(x) - _bessel_series(0, x)) < 1e-14, x
+        assert abs(O.j1(x) - _bessel_series(1, x)) < 1e-14, x
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
