@@ -470,6 +470,29 @@ def test_bessel_functions_are_the_power_series():
         assert abs(O.j1(x) - _bessel_series(1, x)) < 1e-14, x
 
 
+@pytest.mark.parametrize("N", [64, 512, 4096])
+def test_numpy_fft_is_the_dft(N):
+    # the oracle's transforms along t are numpy.fft where the reference calls cupy.fft:
+    # pin numpy's forward and inverse transforms (sign, ordering, 1/N on the inverse) to the
+    # direct DFT summed in long double, with exp(-2 pi i m / N) at m = k n mod N
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    n = np.arange(N)
+    ang = (2 * np.pi / np.longdouble(N)) * n.astype(np.longdouble)
+    w = np.cos(ang) - 1j * np.sin(ang).astype(np.longdouble)
+    ks = [0, 1, 2, N // 3, N // 2, N - 1] if N > 64 else list(range(N))
+    xr, xi = x.real.astype(np.longdouble), x.imag.astype(np.longdouble)
+    for k in ks:
+        m = (k * n) % N
+        c, s_ = w.real[m], w.imag[m]
+        ref = complex(float(np.sum(xr * c - xi * s_)), float(np.sum(xr * s_ + xi * c)))
+        assert abs(np.fft.fft(x)[k] - ref) <= 1e-14 * np.sqrt(N) * np.abs(x).max(), k
+    X = np.fft.fft(x)
+    assert np.abs(np.fft.ifft(X) - x).max() <= 1e-14 * np.abs(x).max()
+    assert np.allclose(np.fft.ifft(X * N)[:3], [np.sum(X * np.exp(2j * np.pi * np.arange(N) * t / N))
+                                                 for t in range(3)], rtol=1e-12, atol=0)
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
