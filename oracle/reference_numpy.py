@@ -23,10 +23,9 @@ marked EXTENSION and default to the reference behaviour:
 Parity status per function (what pins it) is listed in ``PINS`` at the bottom.
 Parity partly unpinned: ``NonlinearOperator.split``'s individual terms on a
 general (multi-frequency) field, pinned only on special fields and through the
-additivity of the terms; and the SciPy-vs-cupy library arithmetic for j0/j1
-beyond x = 62, pinned only through closed forms (PCHIP is pinned to its
-published Fritsch-Carlson slopes, j0/j1 to their power series below 62, the FFT
-to the direct DFT).
+additivity of the terms. The SciPy/NumPy library arithmetic that stands in for
+cupy's is pinned to published methods: PCHIP to the Fritsch-Carlson slopes,
+j0/j1 to their power series and Hankel's expansion, the FFT to the direct DFT.
 """
 
 from __future__ import annotations

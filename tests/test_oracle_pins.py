@@ -493,6 +493,41 @@ def test_numpy_fft_is_the_dft(N):
                                                  for t in range(3)], rtol=1e-12, atol=0)
 
 
+def _bessel_asymptotic(nu, x):
+    """Hankel's expansion (Abramowitz & Stegun 9.2.5-9.2.10): J_nu(x) = sqrt(2 / (pi x))
+    (P cos chi - Q sin chi), chi = x - (nu / 2 + 1 / 4) pi, with P, Q the even and odd terms
+    of sum_k (-1)^floor(k/2) a_k(nu) / x^k, a_k = prod_{j<=k} (4 nu^2 - (2j - 1)^2) / (k! 8^k),
+    summed until the terms fall below 1e-20 (x >= 62). cos chi and sin chi come from cos x
+    and sin x of the float argument, whose range reduction is exact."""
+    mu = 4.0 * nu * nu
+    a, P, Q, k = 1.0, 1.0, 0.0, 0
+    while abs(a) >= 1e-20 or k < 3:
+        k += 1
+        a *= (mu - (2 * k - 1) ** 2) / (k * 8.0 * x)
+        if k % 2 == 0:
+            P += (-1) ** (k // 2) * a
+        else:
+            Q += (-1) ** (k // 2) * a
+    c, s_ = np.cos(x), np.sin(x)
+    r = np.sqrt(0.5)
+    cc, ss = ((c + s_) * r, (s_ - c) * r) if nu == 0 else ((s_ - c) * r, -(s_ + c) * r)
+    return np.sqrt(2.0 / (np.pi * x)) * (P * cc - Q * ss)
+
+
+def test_bessel_functions_at_large_arguments():
+    # the Hankel matrix at N = 1025 evaluates j0 / j1 up to j_N = 3219.3: pin SciPy's values
+    # there to Hankel's asymptotic expansion (the power series pin above stops at 62)
+    z = O.jn_zeros(0, 1025)
+    # McMahon's expansion of the zeros (A&S 9.5.12, nu = 0): b = (s - 1/4) pi,
+    # j_s = b + 1/(8b) - 124/(3 (8b)^3) + 120928/(15 (8b)^5) - ...
+    b = (np.arange(1, 1026) - 0.25) * np.pi
+    mc = b + 1 / (8 * b) - 124 / (3 * (8 * b) ** 3) + 120928 / (15 * (8 * b) ** 5)
+    assert np.abs(z[19:] - mc[19:]).max() < 1e-11
+    for x in np.concatenate([z[19::16], np.linspace(62.0, 3300.0, 97)]):
+        assert abs(O.j0(x) - _bessel_asymptotic(0, x)) < 1e-14, x
+        assert abs(O.j1(x) - _bessel_asymptotic(1, x)) < 1e-14, x
+
+
 def test_alpha_pointwise_closed_form():
     cfg = _small_cfg()
     g = O.build_grids(cfg)
