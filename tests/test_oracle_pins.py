@@ -291,6 +291,51 @@ def test_split_single_frequency_spectral_operators(m, stf):
     assert np.abs(rhs - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("term", ["kerr", "photo", "defoc", "ste"])
+def test_split_term_on_a_general_field_is_its_tone_multiplier(term):
+    # On a tone (constant |u|, constant densities) a term's spectral operator shows as one
+    # complex factor M[m] per frequency bin over its pointwise source: |u|^2 u (Kerr),
+    # rho u (plasma defocusing), rho_s u (STE index) and, for photo-ionisation, photo u,
+    # whose alpha u = photo u part is added unfiltered. On a general field, with random
+    # densities, each term must be exactly the diagonal operator with those factors applied
+    # to its source. An operator applied before the pointwise product agrees on tones and
+    # fails here.
+    on = {"kerr": dict(enable_kerr_instantaneous=True),
+          "photo": dict(enable_photoionization_loss=True),
+          "defoc": dict(enable_plasma_defocusing=True),
+          "ste": dict(enable_ste=True, enable_ste_index=True)}[term]
+    cfg = _small_cfg(**{**_NO_NL, **on, "enable_self_steepening": True,
+                        "enable_space_time_focusing": True, "enable_spectral_filter": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+
+    def source(u, a, rho, rho_s):
+        return {"kerr": np.abs(u) ** 2 * u, "photo": a * u, "defoc": rho * u,
+                "ste": rho_s * u}[term]
+
+    flat = np.full(g["rr"].shape, 1e20)
+    M = np.zeros(cfg.Nt, complex)
+    for m in range(cfg.Nt):
+        u = _tone(g, m, 4e13)
+        rhs, a = nl.split(u, flat, flat)
+        sv = source(u, a, flat, flat)
+        ratio = (sv - rhs) / sv if term == "photo" else rhs / sv
+        M[m] = ratio[0, 0]
+        assert np.allclose(ratio, M[m], rtol=1e-12, atol=1e-12), m     # one factor per tone
+    assert np.ptp(np.abs(M)) > 1e-3 * np.abs(M).max()   # the operators are not a constant
+    u = _field(cfg, g, seed=5, scale=20.0)
+    rng = np.random.default_rng(9)
+    rho, rho_s = rng.uniform(0, 1e20, u.shape), rng.uniform(0, 1e20, u.shape)
+    rhs, a = nl.split(u, rho, rho_s)
+    sv = source(u, a, rho, rho_s)
+    assert np.abs(sv).max() > 0
+    pred = np.fft.ifft(np.fft.fft(sv, axis=1) * M, axis=1)
+    if term == "photo":
+        pred = sv - pred
+    lit = np.abs(u) ** 2 >= 1e-30
+    assert np.abs(rhs[lit] - pred[lit]).max() <= 1e-11 * np.abs(pred[lit]).max()
+
+
 def test_split_raman_of_constant_intensity():
     # Raman alone on a constant |u|^2 = A: the response integral is the DC value of R_f
     # (its closed form is pinned above), rhs = i kappa f_R A Re(R_f[0]) u
