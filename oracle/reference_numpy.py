@@ -21,9 +21,10 @@ marked EXTENSION and default to the reference behaviour:
   seeded complex noise (the workload inputs).
 
 Parity status per function (what pins it) is listed in ``PINS`` at the bottom.
-Parity partly unpinned: the Raman term of ``NonlinearOperator.split`` alone on a
-non-constant intensity, pinned only through the closed form of R_f and the
-constant-intensity case. The SciPy/NumPy library arithmetic that stands in for
+Parity partly unpinned: the Raman term of ``NonlinearOperator.split`` on a
+general intensity, pinned per Fourier component of the intensity (constant and
+single-cosine intensities) through the closed form of R_f. The SciPy/NumPy
+library arithmetic that stands in for
 cupy's is pinned to published methods: PCHIP to the Fritsch-Carlson slopes,
 j0/j1 to their power series and Hankel's expansion, the FFT to the direct DFT.
 """

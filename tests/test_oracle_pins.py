@@ -350,6 +350,29 @@ def test_split_raman_of_constant_intensity():
     assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("m", [3, 11])
+def test_split_raman_of_a_modulated_intensity(m):
+    # Raman alone (no spectral operators) on |u|^2 = a2 (1 + cos(theta n) / 2), u real:
+    # the delayed response Re ifft(fft(|u|^2) R_f) of one cosine is
+    # a2 Re R_f[0] + (a2 / 4) Re(R_f[m] e^(i theta n) + R_f[-m] e^(-i theta n)), so
+    # rhs = i kappa f_R (that) u. Unlike the constant-intensity case this sees R_f away from
+    # DC: a time-reversed (conjugated) response or the imaginary part fails it.
+    cfg = _small_cfg(**{**_NO_NL, "enable_kerr_raman": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    Nt = cfg.Nt
+    th = 2 * np.pi * m * np.arange(Nt) / Nt
+    a2 = 3e13 / g["invE2"]
+    u = np.broadcast_to(np.sqrt(a2 * (1 + 0.5 * np.cos(th))), g["rr"].shape).astype(complex)
+    rhs, a = nl.split(u, np.zeros(u.shape), None)
+    R = g["R_f"]
+    resp = a2 * R[0].real + (a2 / 4) * (R[m] * np.exp(1j * th) + R[-m] * np.exp(-1j * th)).real
+    ref = 1j * _kappa(cfg) * cfg.f_R * resp * u
+    assert np.all(a == 0) and abs(R[m] - np.conj(R[-m])) <= 1e-12 * abs(R[m])  # real response
+    assert abs(np.angle(R[m])) > 1e-2                 # the response is delayed at f_m
+    assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_split_ste_index_term_alone():
     # the STE index change alone: extra = +i ste_pref rho_s u, so rhs = i ste_pref rho_s u
     # (ste_pref itself is pinned by the recorded delta n_STE above)
