@@ -1,6 +1,7 @@
 """CPU tests of the product's host side: the C-ABI library loads and exports
 every declared entry point, and the host setup (Config, Keldysh tables, grids,
 envelopes) equals the oracle's."""
+import ctypes
 import os
 import re
 import subprocess
@@ -142,6 +143,26 @@ def test_record_history_rejects_null_spec():
     args[0] = None
     with pytest.raises(RuntimeError, match="null pointer"):
         _lib.call("fp_record_history", *args)
+
+
+@pytest.mark.parametrize("Nr,Nt,msg", [
+    (200, 512, "Nr % 128"), (0, 512, "Nr % 128"), (128, 100, "2 Nt % 128"),
+])
+def test_mixed_gemm_rejects_ragged_shapes(Nr, Nt, msg):
+    """The split-FP16 GEMM tiles the radius by 128: a ragged radial grid is refused on the
+    host, by the C entry point and by the operand split, before any launch."""
+    from rapport_amd import _lib
+    from rapport_amd.mixed import split_hankel_fp16
+    a = 0x1000
+    with pytest.raises(RuntimeError, match=msg):
+        _lib.call("fp_hankel_gemm_mixed", a, a, 0, a, 0x2000, Nr, Nt, ctypes.c_float(1.0),
+                  None, None, a, None, None, None)
+    with pytest.raises(RuntimeError, match="alias"):
+        _lib.call("fp_hankel_gemm_mixed", a, a, 0, a, a, 128, 512, ctypes.c_float(1.0),
+                  None, None, a, None, None, None)
+    if Nr > 0 and Nr % 128:
+        with pytest.raises(ValueError, match="Nr % 128"):
+            split_hankel_fp16(np.eye(Nr))
 
 
 def test_header_is_a_c_abi(tmp_path):
