@@ -205,6 +205,24 @@ int fp_record_history(const void* spec, int spec_c64, const void* tw, const doub
                       double dt, int nbins, const int* bin_lo, const int* bin_hi,
                       const fp_record_dst* d, void* stream);
 
+/* Largest Nr that fp_record_history_noalloc accepts. */
+#define FP_NOALLOC_MAX_NR 16384
+
+/* fp_record_history without caller scratch: the same kernels and results, with the
+ * 3 x Nr per-row partials in a static device array of the library (3 x
+ * FP_NOALLOC_MAX_NR doubles per device, kept until the process ends); d->partial is
+ * ignored. Has no counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops). It calls
+ * no allocator. Returns FP_E_SHAPE when Nr > FP_NOALLOC_MAX_NR. Uses the array of the
+ * current device, so `stream` must belong to that device. Every call on one device
+ * shares the array: calls that overlap on one device (concurrent streams or host
+ * threads) race; calls ordered on one stream do not (every partial is written before
+ * it is read). */
+int fp_record_history_noalloc(const void* spec, int spec_c64, const void* tw, const double* rho,
+                              const double* rho_s, const fp_nl_params* p, const fp_nl_tables* t,
+                              double dt, int nbins, const int* bin_lo, const int* bin_hi,
+                              const fp_record_dst* d, void* stream);
+
 /* ===========================================================================
  * Mixed-precision path (Config.precision = "mixed"): complex64 fields, FP32
  * transforms and RK4, split-FP16 Hankel GEMM with FP32 accumulation, FP64

@@ -86,6 +86,9 @@ _SIGS = {
                        c_int, P, P, P, P],
     "fp_record_history": [P, c_int, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTables),
                           c_double, c_int, P, P, ctypes.POINTER(RecordDst), P],
+    "fp_record_history_noalloc": [P, c_int, P, P, P, ctypes.POINTER(NLParams),
+                                  ctypes.POINTER(NLTables), c_double, c_int, P, P,
+                                  ctypes.POINTER(RecordDst), P],
     "fp_absmax_f32": [P, ctypes.c_longlong, P, P],
     "fp_fft_rows_c64": [P, P, c_int, c_int, c_int, P, P],
     "fp_plasma_scan_spec_c64": [P, P, P, P, ctypes.POINTER(PlasmaParams), P, P, P, P, P, P],

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -547,6 +548,48 @@ int dispatch_record_history(const void* spec, const void* tw, const double* rho,
   }
 }
 
+int record_history(const void* spec, int spec_c64, const void* tw, const double* rho,
+                   const double* rho_s, const fp_nl_params* p, const fp_nl_tables* t, double dt,
+                   int nbins, const int* bin_lo, const int* bin_hi, const fp_record_dst& d,
+                   hipStream_t st);
+int record_check(const void* spec, const void* tw, const double* rho, const fp_nl_params* p,
+                 const fp_nl_tables* t, int nbins, const int* bin_lo, const int* bin_hi,
+                 const fp_record_dst& dd, bool need_partial);
+
+// Scratch of fp_record_history_noalloc: the 3 x Nr FP64 per-row partials of one record
+// (3 x 16384 x 8 B = 384 KiB per device, held until the process ends). In host code
+// `scratch` is only a stand-in symbol: its device address comes from hipGetSymbolAddress,
+// once per device, and only that address is passed to kernels.
+static __device__ __align__(256) double scratch[3 * FP_NOALLOC_MAX_NR];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call there).
+int record_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    g_err = std::string("fp_record_history_noalloc: hipGetDevice: ") + hipGetErrorString(e);
+    return (int)e;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return fail(FP_E_ARG, "fp_record_history_noalloc: device index");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* a = nullptr;
+    e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a) {
+      g_err = std::string("fp_record_history_noalloc: hipGetSymbolAddress: ") +
+              hipGetErrorString(e);
+      return e != hipSuccess ? (int)e : FP_E_ARG;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(a);
+  }
+  *out = g_scratch_dev[dev];
+  return FP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -658,12 +701,44 @@ int fp_record_history(const void* spec, int spec_c64, const void* tw, const doub
                       const double* rho_s, const fp_nl_params* p, const fp_nl_tables* t,
                       double dt, int nbins, const int* bin_lo, const int* bin_hi,
                       const fp_record_dst* d, void* stream) {
-  if (!spec || !tw || !rho || !p || !t || !d)
+  if (!d) return fail(FP_E_ARG, "fp_record_history: null pointer");
+  if (!d->partial) return fail(FP_E_ARG, "fp_record_history: null destination");
+  return record_history(spec, spec_c64, tw, rho, rho_s, p, t, dt, nbins, bin_lo, bin_hi, *d,
+                        S(stream));
+}
+
+int fp_record_history_noalloc(const void* spec, int spec_c64, const void* tw, const double* rho,
+                              const double* rho_s, const fp_nl_params* p, const fp_nl_tables* t,
+                              double dt, int nbins, const int* bin_lo, const int* bin_hi,
+                              const fp_record_dst* d, void* stream) {
+  if (!d || !p) return fail(FP_E_ARG, "fp_record_history_noalloc: null pointer");
+  if (p->Nr > FP_NOALLOC_MAX_NR)
+    return fail(FP_E_SHAPE, "fp_record_history_noalloc: Nr exceeds FP_NOALLOC_MAX_NR (16384)");
+  int rc = record_check(spec, tw, rho, p, t, nbins, bin_lo, bin_hi, *d, false);
+  if (rc != FP_OK) return rc;
+  double* partial = nullptr;
+  if ((rc = record_scratch(&partial)) != FP_OK) return rc;
+  fp_record_dst dl = *d;
+  dl.partial = partial;
+  return record_history(spec, spec_c64, tw, rho, rho_s, p, t, dt, nbins, bin_lo, bin_hi, dl,
+                        S(stream));
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host-side validation of a record's arguments (no device call).
+int record_check(const void* spec, const void* tw, const double* rho, const fp_nl_params* p,
+                 const fp_nl_tables* t, int nbins, const int* bin_lo, const int* bin_hi,
+                 const fp_record_dst& dd, bool need_partial) {
+  const fp_record_dst* d = &dd;
+  if (!spec || !tw || !rho || !p || !t)
     return fail(FP_E_ARG, "fp_record_history: null pointer");
   if (p->Nr <= 0) return fail(FP_E_SHAPE, "fp_record_history: Nr must be positive");
   if (!d->fluence || !d->rho || !d->rho_s || !d->ipeak || !d->absorbed || !d->rho_onaxis ||
-      !d->rho_s_onaxis || !d->I_onaxis || !d->r_w || !d->partial || !d->imax ||
-      !d->dedz_photo || !d->dedz_plasma)
+      !d->rho_s_onaxis || !d->I_onaxis || !d->r_w || (need_partial && !d->partial) ||
+      !d->imax || !d->dedz_photo || !d->dedz_plasma)
     return fail(FP_E_ARG, "fp_record_history: null destination");
   if (nbins < 0 || (nbins > 0 && (!bin_lo || !bin_hi || !d->bins)))
     return fail(FP_E_ARG, "fp_record_history: bins");
@@ -677,7 +752,18 @@ int fp_record_history(const void* spec, int spec_c64, const void* tw, const doub
     if (!d->rho_cube || !d->rho_s_cube || !d->I_cube)
       return fail(FP_E_ARG, "fp_record_history: null cube destination");
   }
-  const hipStream_t st = S(stream);
+  if (p->Nt != 512 && p->Nt != 1024 && p->Nt != 2048 && p->Nt != 4096)
+    return fail(FP_E_SHAPE, "fp_record_history: Nt must be 512, 1024, 2048 or 4096");
+  return FP_OK;
+}
+
+int record_history(const void* spec, int spec_c64, const void* tw, const double* rho,
+                   const double* rho_s, const fp_nl_params* p, const fp_nl_tables* t, double dt,
+                   int nbins, const int* bin_lo, const int* bin_hi, const fp_record_dst& dd,
+                   hipStream_t st) {
+  const fp_record_dst* d = &dd;
+  int rc0 = record_check(spec, tw, rho, p, t, nbins, bin_lo, bin_hi, dd, true);
+  if (rc0 != FP_OK) return rc0;
   const int rc = spec_c64 ? dispatch_record_history<float>(spec, tw, rho, rho_s, *p, *t, dt, nbins,
                                                            bin_lo, bin_hi, *d, st)
                           : dispatch_record_history<double>(spec, tw, rho, rho_s, *p, *t, dt,
@@ -687,6 +773,10 @@ int fp_record_history(const void* spec, int spec_c64, const void* tw, const doub
                      *d);
   return check_launch("record_finish");
 }
+
+}  // namespace
+
+extern "C" {
 
 // ------------------------------------------------------------------ mixed precision
 // The FP32 PCHIP table is read one float4 per interval.

@@ -137,6 +137,22 @@ def test_record_history_rejects_bad_arguments(kw, msg):
         _lib.call("fp_record_history", *args)
 
 
+@pytest.mark.parametrize("kw,msg", [
+    (dict(Nr=16385), "FP_NOALLOC_MAX_NR"),
+    (dict(Nr=0), "Nr must be positive"),
+    (dict(fluence=None), "null destination"),
+    (dict(t_stride=7, r_stride=3, Nt_sub=73), "Nt_sub"),
+])
+def test_record_history_noalloc_rejects_bad_arguments(kw, msg):
+    """The allocation-free record has the same host checks, plus its row limit (the
+    static scratch holds 3 x 16384 partials); none of these reaches the GPU."""
+    from rapport_amd import _lib
+    shape = {k: kw.pop(k) for k in ("Nr", "Nt", "nbins") if k in kw}
+    args = _record_args(**shape, **kw)
+    with pytest.raises(RuntimeError, match=msg):
+        _lib.call("fp_record_history_noalloc", *args)
+
+
 def test_record_history_rejects_null_spec():
     from rapport_amd import _lib
     args = _record_args()
