@@ -20,13 +20,11 @@ marked EXTENSION and default to the reference behaviour:
 * ``envelope_lens_noise``: collimated Gaussian with a thin-lens phase and
   seeded complex noise (the workload inputs).
 
-Parity status per function (what pins it) is listed in ``PINS`` at the bottom.
-Parity partly unpinned: the Raman term of ``NonlinearOperator.split`` on a
-general intensity, pinned per Fourier component of the intensity (constant and
-single-cosine intensities) through the closed form of R_f. The SciPy/NumPy
-library arithmetic that stands in for
-cupy's is pinned to published methods: PCHIP to the Fritsch-Carlson slopes,
-j0/j1 to their power series and Hankel's expansion, the FFT to the direct DFT.
+Parity status per function (what pins it) is listed in ``PINS`` at the bottom;
+every function there has a pin that is not a restatement of itself. The
+SciPy/NumPy library arithmetic that stands in for cupy's is pinned to published
+methods: PCHIP to the Fritsch-Carlson slopes, j0/j1 to their power series and
+Hankel's expansion, the FFT to the direct DFT.
 """
 
 from __future__ import annotations
@@ -524,20 +522,18 @@ def exact_exp_step(x, S, L, dt):
     return np.where(~np.isfinite(xn) | (xn < 0.0), 0.0, xn)
 
 
-def rate_equation_rows(E, ne, ns, dt, field_to_I, beta_g, beta_s, na, inv_tau_r,
-                       inv_tau_ste, enable_ste, W_val, W_ste, size, log_min, inv_step,
-                       Imin, Imax):
-    """solve_rate_equation_kernel, kernels.py:36-106: one row per radius,
-    sequential in t; rows advance together. Writes ne[:, 1:], ns[:, 1:]."""
-    Nr, Nt = E.shape
+def rate_recurrence_rows(I_all, W_all, Ws_all, ne, ns, dt, beta_g, beta_s, na, inv_tau_r,
+                         inv_tau_ste, enable_ste):
+    """The per-thread t-loop of solve_rate_equation_kernel (kernels.py:59-104) on given
+    per-sample intensities I_all and rates W_all, Ws_all (Nr x Nt): sanitised start
+    values (:53-55), sample averages (:64-70), depletion (:73-75), the coupled exact
+    exponential steps (:77-88) and the n_e + n_s <= n_a rescaling (:91-95). Writes
+    ne[:, 1:], ns[:, 1:]; ne[:, 0], ns[:, 0] are the start values."""
+    Nt = I_all.shape[1]
     ne_v = ne[:, 0].copy()
     ns_v = ns[:, 0].copy()
     ne_v = np.where(~np.isfinite(ne_v) | (ne_v < 0.0), 0.0, ne_v)    # :53-55
     ns_v = np.where(~np.isfinite(ns_v) | (ns_v < 0.0), 0.0, ns_v)
-    I_all = (E.real ** 2 + E.imag ** 2) * field_to_I                  # f.x*f.x + f.y*f.y
-    lut = (size, log_min, inv_step, Imin, Imax)
-    W_all = lut_rate(I_all, W_val, *lut)
-    Ws_all = lut_rate(I_all, W_ste, *lut) if enable_ste else np.zeros_like(I_all)
     for k in range(Nt - 1):                                           # :62
         I_c, I_n = I_all[:, k], I_all[:, k + 1]
         Ia = 0.5 * (I_c + I_n)
@@ -565,6 +561,19 @@ def rate_equation_rows(E, ne, ns, dt, field_to_I, beta_g, beta_s, na, inv_tau_r,
         ne[:, k + 1] = ne_n
         ns[:, k + 1] = ns_n
         ne_v, ns_v = ne_n, ns_n
+
+
+def rate_equation_rows(E, ne, ns, dt, field_to_I, beta_g, beta_s, na, inv_tau_r,
+                       inv_tau_ste, enable_ste, W_val, W_ste, size, log_min, inv_step,
+                       Imin, Imax):
+    """solve_rate_equation_kernel, kernels.py:36-106: one row per radius,
+    sequential in t; rows advance together. Writes ne[:, 1:], ns[:, 1:]."""
+    I_all = (E.real ** 2 + E.imag ** 2) * field_to_I                  # f.x*f.x + f.y*f.y
+    lut = (size, log_min, inv_step, Imin, Imax)
+    W_all = lut_rate(I_all, W_val, *lut)                              # :64-70
+    Ws_all = lut_rate(I_all, W_ste, *lut) if enable_ste else np.zeros_like(I_all)
+    rate_recurrence_rows(I_all, W_all, Ws_all, ne, ns, dt, beta_g, beta_s, na, inv_tau_r,
+                         inv_tau_ste, enable_ste)
 
 
 # --------------------------------------------------------------------------
@@ -772,13 +781,24 @@ PINS = {
                                "alpha u, Drude index, Raman Re R_f[0], STE index, T^2 / T / "
                                "1/U placement); zero-field mask; alpha clip; on a general "
                                "field the all-terms rhs and alpha are the sums of the "
-                               "one-term runs",
-    "rate_equation_rows": "constant-I exact solution of dx/dt=S+Lx; independent scalar "
+                               "one-term runs; Raman alone on a general intensity = the "
+                               "direct time-domain convolution with the sampled kernel",
+    "Keldysh.rate(monotone=True)": "non-decreasing over 1e11.5..1e14.2; at its knots the "
+                                   "running maximum of rate_raw, equal to rate_raw where "
+                                   "rate_raw sets a new maximum, above it in the cusps",
+    "rate_recurrence_rows": "input/output vectors of the reference's own integre_0d (STE on, "
+                            "full / no avalanche / no trapping, reaching the n_e + n_s = n_a "
+                            "rescaling) to 1e-9",
+    "rate_equation_rows": "constant-I exact solution of dx/dt=S+Lx; L = 0 depletion closed "
+                          "form n_a (1 - (1 - W dt / n_a)^k); independent scalar "
                           "restatement; non-negativity and rho<=rho_max; lut_rate at its "
                           "knots and at i + 3/4 (truncating index, 4095 log10 steps)",
     "Integrator": "linear-only run = linear Gaussian propagation; nz+1 steps, z labels; "
                   "record invariants (bins sum, rho_at<=rho_rz, max Ipeak = max Imax); "
                   "result.npz size estimate 0.99 GB / 86% cube at the 1030 nm config; "
                   "radial absorber mask_r; two alpha half-steps = one absorption length; "
-                  "outputs mirrored about r = 0",
+                  "outputs mirrored about r = 0; energy bookkeeping (dE/dz, absorbed, "
+                  "E_*_z) balances the recorded field energy: plasma absorption on a "
+                  "uniform density to 1e-4, photo-ionisation to 1e-2 (x2, x100 and 2 pi r "
+                  "slips fail)",
 }

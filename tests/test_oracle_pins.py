@@ -373,6 +373,39 @@ def test_split_raman_of_a_modulated_intensity(m):
     assert np.abs(rhs - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+def test_split_raman_of_a_general_intensity_is_the_time_domain_convolution():
+    # Raman alone (no spectral operators) on a general field: the response is the circular
+    # convolution of |u|^2 with the sampled causal kernel h[k] = Omega_r^2 tau_s dt
+    # exp(-k dt / tau_d) sin(k dt / tau_s), k = 1 .. N/2 - 1 (the lags ifftshift puts at
+    # t > 0), summed here directly in the time domain, with no FFT: rhs = i kappa f_R
+    # (h * |u|^2) u. Pins the whole Raman path on arbitrary intensities (orientation,
+    # lag range, the dt factor and the real part), not only per Fourier component.
+    cfg = _small_cfg(N=9, **{**_NO_NL, "enable_kerr_raman": True})
+    g = O.build_grids(cfg)
+    nl = O.NonlinearOperator(cfg, g)
+    Nr, Nt = g["rr"].shape
+    rng = np.random.default_rng(11)
+    a2 = 3e13 / g["invE2"]
+    amp = np.sqrt(a2 * rng.uniform(0.0, 1.0, (Nr, Nt)))
+    u = amp * np.exp(2j * np.pi * rng.uniform(size=(Nr, Nt)))
+    rhs, a = nl.split(u, np.zeros(u.shape), None)
+    dt = g["dt"]
+    k = np.arange(1, Nt // 2)
+    h = cfg.Omega_r2 * cfg.tau_s * dt * np.exp(-k * dt / cfg.tau_d) * np.sin(k * dt / cfg.tau_s)
+    I = np.abs(u) ** 2
+    resp = np.zeros_like(I)
+    for j, kk in enumerate(k):
+        resp += h[j] * np.roll(I, kk, axis=1)          # I[n - kk]
+    ref = 1j * _kappa(cfg) * cfg.f_R * resp * u
+    assert np.all(a == 0)
+    assert np.abs(rhs - ref).max() <= 1e-11 * np.abs(ref).max()
+    # a time-reversed kernel (I[n + k]) is a different operator on this field
+    rev = np.zeros_like(I)
+    for j, kk in enumerate(k):
+        rev += h[j] * np.roll(I, -kk, axis=1)
+    assert np.abs(rev - resp).max() > 1e-2 * np.abs(resp).max()
+
+
 def test_split_ste_index_term_alone():
     # the STE index change alone: extra = +i ste_pref rho_s u, so rhs = i ste_pref rho_s u
     # (ste_pref itself is pinned by the recorded delta n_STE above)
@@ -791,3 +824,155 @@ def test_record_invariants():
     assert np.isclose(res["Ipeak_rz"].max(), res["Imax_z"].max(), rtol=1e-6)
     assert res["rho_rz"].max() > 0
     assert res["rho_rzt"].shape == (3, 32, 43)
+
+
+def test_rate_recurrence_reproduces_the_reference_0d_vectors():
+    # The carrier kernel's recurrence (kernels.py:59-104) and the reference's own 0-D
+    # re-integration (figures_article.py:161-194) are the same two-population scheme: with
+    # inv_tau_ste = 0 and the rates evaluated at the samples they differ only in
+    # expm1(x)/x against (exp(x)-1)/x. The reference-produced vectors (tests/golden/
+    # rate0d_ste.npz: STE on, all three avalanche / trapping variants, reaching the
+    # n_e + n_s = n_a rescaling) therefore pin the STE branch, the depletion factor and
+    # the rescaling of the oracle's recurrence.
+    import os
+    from oracle import postprocess_numpy as P0
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "rate0d_ste.npz"))
+    prm = P0.Params0D()
+    I = np.clip(gold["I"], 0.0, None)
+    W, Ws = prm.W_vb(I), prm.W_ste(I)
+    dt = (gold["t_fs"][1] - gold["t_fs"][0]) * 1e-15
+    assert np.allclose(np.diff(gold["t_fs"]) * 1e-15, dt, rtol=1e-12, atol=0)
+    for name, bg, itr in (("full", prm.beta_g, 1.0 / prm.tau_r),
+                          ("noav", 0.0, 1.0 / prm.tau_r),
+                          ("notrap", prm.beta_g, 0.0)):
+        ne, ns = np.zeros_like(I), np.zeros_like(I)
+        O.rate_recurrence_rows(I, W, Ws, ne, ns, dt, bg, prm.beta_s, prm.rho_max, itr, 0.0, True)
+        scale = max(np.abs(gold["ne_" + name]).max(), np.abs(gold["ns_" + name]).max())
+        assert np.abs(ne - gold["ne_" + name]).max() <= 1e-9 * scale, name
+        assert np.abs(ns - gold["ns_" + name]).max() <= 1e-9 * scale, name
+    assert np.max(gold["ne_full"] + gold["ns_full"]) >= prm.rho_max * (1 - 1e-12)
+
+
+def test_rate_equation_rows_depletion_closed_form():
+    # With no avalanche and no trapping (L = 0) the exact-exponential step is n + S dt, and
+    # the depletion factor gives n_{k+1} = n_k + W dt (1 - n_k / n_a): the closed form
+    # n_a (1 - (1 - W dt / n_a)^k), through rate_equation_rows itself (field -> intensity
+    # -> rate lookup -> recurrence). Fails with the depletion factor dropped or taken from
+    # the new density, or with the average of the wrong samples.
+    cfg = _small_cfg(Nt=512, N=3)
+    g = O.build_grids(cfg)
+    kel = g["keldysh"]
+    lut = (kel["NLUT"], kel["logImin"], kel["inv_dlog"], kel["Imin"], kel["Imax"])
+    I0 = 8e13
+    E = np.full((2, cfg.Nt), np.sqrt(I0 / g["invE2"]), complex)
+    W = O.lut_rate(np.array([I0]), kel["W_LUT"], *lut)[0]
+    dt, na = g["dt"], 2.1e22
+    a = W * dt / na
+    k = np.arange(cfg.Nt)
+    assert a * cfg.Nt > 1.0                                       # depletion really acts
+    ne, ns = np.zeros(E.shape), np.zeros(E.shape)
+    O.rate_equation_rows(E, ne, ns, dt, g["invE2"], 0.0, 0.0, na, 0.0, 0.0, False,
+                         kel["W_LUT"], kel["W_LUT_s"], *lut)
+    exact = na * -np.expm1(k * np.log1p(-a))
+    assert np.abs(ne[0] - exact).max() <= 1e-11 * na
+    assert not ns.any()
+
+
+def _balance_run(dz=20e-6, n=31, rho0=1e17, w0=100e-6, R_factor=10, energy_uJ=2.0, **toggles):
+    """n steps at a frozen uniform density, each followed by a record (integrator.py:
+    138-149, 179-224); returns the integrator and its results."""
+    cfg = _small_cfg(nz=n - 1, begin=0.0, end=(n - 1) * dz, N=129, Nt=128, R_factor=R_factor,
+                     w0=w0, energy_uJ=energy_uJ, rho_t_stride=0, save_stride=1,
+                     **{**_NO_NL, **toggles})
+    it = O.Integrator(cfg)
+    for i in range(n):
+        it.rho[:] = rho0
+        it.step(it.dz)
+        it._record(cfg.begin + i * it.dz)
+    return it, it._results()
+
+
+def _energies_uJ(it, fluence_rz):
+    r = it.g["rlist"]
+    w = 2.0 * np.pi * r * np.diff(r, prepend=0.0)
+    return (fluence_rz[:, r.size:].astype(np.float64) * w).sum(axis=1) * 1e10
+
+
+def _trapz_over_U0(it, d):
+    z = it.z_saved[:it.k_save]
+    return np.concatenate([[0.0], np.cumsum(0.5 * (d[:-1] + d[1:]) * np.diff(z))]) / it.U0_uJ
+
+
+def test_energy_bookkeeping_balances_the_field_energy():
+    # integrator.py:199-212, 226-236. With T = U^-1 = 1 the loss terms of the RK4 rhs
+    # cancel alpha u, so the field loses energy only through the alpha half-steps. The
+    # bookkeeping must then account for what the field lost: the cumulative E_*_z
+    # (trapezoid of dE/dz over z, over U0) equals (U(z_0) - U(z_k)) / U0 with U from the
+    # recorded fluence (sum_r F 2 pi r dr, J/cm^2 x cm^2 -> uJ). Plasma absorption on a
+    # uniform density (alpha uniform): dE/dz = 2 alpha U and absorbed = 2 alpha 100 F per
+    # r, and the balance holds to the trapezoid error; photo-ionisation (alpha peaked in
+    # r and t): to the O(dz) error of evaluating the loss at the records.
+    it, res = _balance_run(enable_plasma_absorption=True)
+    alpha = it.g["sigmaomega"] / 2 * 100 * 1e17
+    U = _energies_uJ(it, res["fluence_rz"])
+    n = it.k_save
+    d = it._dEdz_plasma_uJm[:n]
+    bal = (U[0] - U) / it.U0_uJ
+    E = res["E_plasma_z"]
+    assert bal[-1] > 0.02                                    # a visible loss
+    assert np.abs(d - 2 * alpha * U).max() <= 1e-6 * d.max()
+    F = res["fluence_rz"][:, it.g["rlist"].size:].astype(np.float64)
+    ab = res["absorbed_rz"][:, it.g["rlist"].size:]
+    assert np.abs(ab - 2 * alpha * 100 * F).max() <= 1e-6 * ab.max()
+    assert np.abs(E - bal).max() <= 1e-4 * bal[-1]
+    assert np.array_equal(res["E_total_z"], res["E_MPI_z"] + res["E_plasma_z"])
+    assert not res["E_MPI_z"].any()
+    # the same checks reject each plausible slip in the bookkeeping
+    r = it.g["rlist"]
+    for bad in (2.0 * d,                                               # x2 on dE/dz
+                d / (2.0 * np.pi * r).mean(),                          # no 2 pi r weight
+                d * 100.0):                                            # x100 on dE/dz
+        assert np.abs(_trapz_over_U0(it, bad) - bal).max() > 0.1 * bal[-1]
+    assert np.abs(ab / 100.0 - 2 * alpha * 100 * F).max() > 0.5 * ab.max() / 100.0
+
+    it, res = _balance_run(enable_photoionization_loss=True, w0=20e-6, R_factor=4, energy_uJ=10.0)
+    U = _energies_uJ(it, res["fluence_rz"])
+    bal = (U[0] - U) / it.U0_uJ
+    E = res["E_MPI_z"]
+    assert bal[-1] > 5e-3 and not res["E_plasma_z"].any()
+    assert abs(E[-1] - bal[-1]) <= 1e-2 * bal[-1]
+    # absorbed_rz and dE/dz are the same per-row loss integral (x100, and x 2 pi r dr 1e10)
+    r = it.g["rlist"]
+    ab = res["absorbed_rz"][:, r.size:].astype(np.float64)
+    w = 2.0 * np.pi * r * np.diff(r, prepend=0.0)
+    d = it._dEdz_photo_uJm[:it.k_save]
+    assert np.abs((ab / 100.0 * w).sum(axis=1) * 1e10 - d).max() <= 1e-6 * d.max()
+
+
+def test_keldysh_monotone_envelope():
+    # keldysh.py:132-148: rate(monotone=True) is the PCHIP (in log10 I) of the running
+    # maximum of rate_raw on 6000 points over 1e0..1e17 W/cm^2. PCHIP of monotone data is
+    # monotone and interpolates its knots, so: non-decreasing everywhere; at each knot the
+    # running maximum (hence >= rate_raw there); equal to rate_raw wherever rate_raw sets a
+    # new maximum; well above rate_raw inside the cusps where rate_raw decreases (23 such
+    # samples over 1e11.5..1e14.2, recorded by the reference's own validation).
+    n0 = O.n_sellmeier(800e-9)
+    kel = O.Keldysh(800e-9, 9.0, 0.64 * m_e, n0, monotone=True)
+    I = np.logspace(11.5, 14.2, 20000)
+    W = kel.rate(I)
+    assert np.all(np.diff(W) >= 0.0)
+    lI = np.linspace(0.0, 17.0, 6000)
+    raw = np.maximum(kel.rate_raw(10.0 ** lI), 0.0)
+    Wk = kel.rate(10.0 ** lI)
+    sel = (lI >= 11.0) & (lI <= 14.5)
+    run = np.array([raw[sel][:j + 1].max() for j in range(sel.sum())])
+    assert np.allclose(Wk[sel], run, rtol=1e-10, atol=0)
+    assert np.all(Wk[sel] >= raw[sel] * (1 - 1e-12))
+    new_max = np.concatenate([[True], raw[sel][1:] > run[:-1]])
+    assert np.allclose(Wk[sel][new_max], raw[sel][new_max], rtol=1e-10, atol=0)
+    Ir = kel.rate_raw(I)
+    dec = np.diff(np.log(np.maximum(Ir, 1e-300))) < 0
+    assert dec.sum() > 0
+    assert np.any(W[1:][dec] > Ir[1:][dec] * 1.01)
+    # the default (monotone=False) is rate_raw itself
+    assert np.array_equal(O.Keldysh(800e-9, 9.0, 0.64 * m_e, n0).rate(I), Ir)
