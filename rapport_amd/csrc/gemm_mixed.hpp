@@ -43,11 +43,18 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int BK = 32, NTHR = 256;
 
-FP_DEV float pow2f(int e) { return __int_as_float((e + 127) << 23); }
+// 2^e as a float for any int e: a normal number for -126 <= e <= 127, subnormal down to
+// 2^-149, 0 below that and +inf above 127 (host and device).
+__host__ __device__ inline float pow2f(int e) {
+  if (e > 127) return __builtin_bit_cast(float, 0x7f800000u);
+  if (e >= -126) return __builtin_bit_cast(float, (unsigned)(e + 127) << 23);
+  if (e >= -149) return __builtin_bit_cast(float, 1u << (e + 149));
+  return 0.0f;
+}
 
 // Power-of-two scale s with max * s < 2^13 (max >= 0); 1 when max is 0 / not finite.
-FP_DEV int scale_exp(float mx) {
-  if (!(mx > 0.0f) || !fp_isfinite(mx)) return 0;
+__host__ __device__ inline int scale_exp(float mx) {
+  if (!(mx > 0.0f) || !__builtin_isfinite(mx)) return 0;
   int e;
   frexpf(mx, &e);           // mx < 2^e
   int se = 13 - e;

@@ -643,6 +643,7 @@ int fp_half_linear(const double* u, double* out, int Nr, int Nt, const double* Y
 int fp_nl_split(const double* u, const double* rho, const double* rho_s, double* rhs,
                 double* alpha, const fp_nl_params* p, const fp_nl_tables* t, void* stream) {
   if (!u || !rho || !rhs || !p || !t) return fail(FP_E_ARG, "fp_nl_split: null pointer");
+  if (p->Nr <= 0) return fail(FP_E_SHAPE, "fp_nl_split: Nr must be positive");
   hipStream_t st = S(stream);
 #define FP_SPLIT(NT)                                                                            \
   if (rho_s)                                                                                    \
@@ -666,6 +667,7 @@ int fp_nl_step(const double* in, double* out, const double* rho, const double* r
                int in_spectral, int out_spectral, const fp_nl_params* p, const fp_nl_tables* t,
                void* stream) {
   if (!in || !out || !rho || !p || !t) return fail(FP_E_ARG, "fp_nl_step: null pointer");
+  if (p->Nr <= 0) return fail(FP_E_SHAPE, "fp_nl_step: Nr must be positive");
   hipStream_t st = S(stream);
 #define FP_STEP(NT)                                                                             \
   if (rho_s)                                                                                    \
@@ -691,6 +693,7 @@ int fp_record_rows(const double* u, const double* rho, const double* rho_s, doub
                    const fp_nl_params* p, const fp_nl_tables* t, double dt, int nbins,
                    const int* bin_lo, const int* bin_hi, double* bins, void* stream) {
   if (!u || !rho || !rows_out || !p || !t) return fail(FP_E_ARG, "fp_record_rows: null pointer");
+  if (p->Nr <= 0) return fail(FP_E_SHAPE, "fp_record_rows: Nr must be positive");
   if (nbins > 0 && (!bin_lo || !bin_hi || !bins)) return fail(FP_E_ARG, "fp_record_rows: bins");
   hipLaunchKernelGGL((fpnl::record_kernel<256>), dim3(p->Nr), dim3(256), 0, S(stream), u, rho,
                      rho_s, rows_out, *p, *t, dt, nbins, bin_lo, bin_hi, bins);
@@ -840,6 +843,7 @@ int fp_nl_split_c64(const float* u, const double* rho, const double* rho_s, floa
                     float* alpha, const fp_nl_params* p, const fp_nl_tables_f32* t,
                     void* stream) {
   if (!u || !rho || !rhs || !p || !t) return fail(FP_E_ARG, "fp_nl_split_c64: null pointer");
+  if (p->Nr <= 0) return fail(FP_E_SHAPE, "fp_nl_split_c64: Nr must be positive");
   if (pcoef_misaligned(t)) return fail(FP_E_ARG, "fp_nl_split_c64: pcoef_s must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define FP_SPLIT32(NT)                                                                           \
@@ -864,6 +868,7 @@ int fp_nl_step_c64(const float* in, float* out, const double* rho, const double*
                    int in_spectral, int out_spectral, const fp_nl_params* p,
                    const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream) {
   if (!in || !out || !rho || !p || !t) return fail(FP_E_ARG, "fp_nl_step_c64: null pointer");
+  if (p->Nr <= 0) return fail(FP_E_SHAPE, "fp_nl_step_c64: Nr must be positive");
   if (pcoef_misaligned(t)) return fail(FP_E_ARG, "fp_nl_step_c64: pcoef_s must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define FP_STEP32(NT)                                                                            \

@@ -26,6 +26,7 @@ from __future__ import annotations
 import ctypes
 import json
 import os
+import weakref
 from typing import Callable, Union
 
 import numpy as np
@@ -61,6 +62,7 @@ class Integrator:
         self.Nr, self.Nt = u0.shape
         u0d = torch.from_numpy(u0).to(dev)
         self._spec = None                       # FFT_t of the field (persistent buffer)
+        self._u_last = None                     # (weakref, version) of the last `u` handed out
         self.rho = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
         self.rho_s = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
         self.mask_r = self.t.mask_r
@@ -160,22 +162,39 @@ class Integrator:
     @property
     def u(self):
         """Time-domain field (Nr x Nt complex128 on the GPU; the mixed path upcasts
-        its complex64 state). A new tensor each call."""
+        its complex64 state), a new tensor each call. As with the reference's array
+        (integrator.py:40), writing into it in place (``it.u[...] = x``) sets the field
+        that the next step, record or propagate() starts from."""
+        self._sync_user_edit()
         if self._mx is not None:
-            return self._mx.time_c64().to(torch.complex128)
-        out = torch.empty_like(self._spec)
-        call("fp_fft_rows", ptr(self._spec), ptr(out), self.Nr, self.Nt, 1, ptr(self.t.tw),
-             stream_ptr())
+            out = self._mx.time_c64().to(torch.complex128)
+        else:
+            out = torch.empty_like(self._spec)
+            call("fp_fft_rows", ptr(self._spec), ptr(out), self.Nr, self.Nt, 1, ptr(self.t.tw),
+                 stream_ptr())
+        self._u_last = (weakref.ref(out), out._version)
         return out
 
     @u.setter
     def u(self, value):
+        self._u_last = None
         if self._mx is not None:
             self._mx.set_time(value)
             return
         v = value.to(torch.complex128).contiguous()
         call("fp_fft_rows", ptr(v), ptr(self._spec), self.Nr, self.Nt, 0, ptr(self.t.tw),
              stream_ptr())
+
+    def _sync_user_edit(self):
+        """Push an in-place edit of the last tensor ``u`` returned into the state."""
+        last = self._u_last
+        if last is None:
+            return
+        t = last[0]()
+        if t is None:
+            self._u_last = None
+        elif t._version != last[1]:
+            self.u = t
 
     def _spectral(self):
         return self._spec
@@ -186,6 +205,7 @@ class Integrator:
     # ------------------------------------------------------------------ one step
     def step(self, dz):
         """Integrator.step (integrator.py:138-149) on the time-domain field."""
+        self._sync_user_edit()
         if self._mx is not None:
             self._mx.strang(dz, self.rho, self._rho_s_arg(), self.nl.params(dz))
             return
@@ -197,6 +217,7 @@ class Integrator:
 
     def _pipeline_step(self, dz):
         """One propagate() iteration body (plasma + step) on the spectral state."""
+        self._sync_user_edit()
         if self._mx is not None:
             self._mx.step(dz, self.rho, self.rho_s, self._rho_s_arg(), self.nl.params(dz))
             return
@@ -227,6 +248,7 @@ class Integrator:
         into a HIP graph on a side stream and return it; graph.replay() then advances
         the persistent state by that many steps. The capture itself does not advance
         the state (one eager warm-up step runs first and allocates the work buffers)."""
+        self._sync_user_edit()
         snap = self._snapshot()
         side = torch.cuda.Stream(device=self.device)
         side.wait_stream(torch.cuda.current_stream(self.device))
@@ -302,6 +324,7 @@ class Integrator:
         """integrator.py:179-224 as one fp_record_history call: the inverse transform of
         the spectral state, the reductions of loss_rates (operators.py:148-163) and every
         history entry of record k are written on the device."""
+        self._sync_user_edit()
         k = self.k_save
         if not 0 <= k < self.n_saves:
             raise IndexError(f"record {k}: the histories hold {self.n_saves} records")

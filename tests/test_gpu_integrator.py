@@ -233,3 +233,26 @@ def test_run_writes_reference_outputs(tmp_path):
     assert set(params) == PARAMS_KEYS and set(params["toggles"]) == TOGGLE_KEYS
     assert params["toggles"]["enable_ste"] is True
     assert abs(params["n0"] - json.load(open(tmp_path / "cpu" / "params.json"))["n0"]) == 0.0
+
+
+def test_in_place_edit_of_u_sets_the_field():
+    # the reference's Integrator.u is an array (integrator.py:40): an in-place write into it
+    # is the field the next step starts from. Same here for the tensor `u` returns.
+    import rapport_amd as R
+    _announce("in-place edit of Integrator.u")
+    kw = cfg_kwargs(C1, nz=4, end=4 * 10e-6)
+    env = envelope_args(C1)
+    for prec in ("fp64", "mixed"):
+        a = R.Integrator(R.Config(**kw, precision=prec), envelope=R.envelope_lens_noise(**env))
+        b = R.Integrator(R.Config(**kw, precision=prec), envelope=R.envelope_lens_noise(**env))
+        ua = a.u
+        ua[:, : a.Nt // 2] *= 0.5                    # in place
+        ub = b.u
+        ub[:, : b.Nt // 2] *= 0.5
+        b.u = ub                                     # explicit set
+        assert torch.equal(a.u, b.u)
+        ra, rb = a.propagate(), b.propagate()
+        assert np.array_equal(ra["fluence_rz"], rb["fluence_rz"]), prec
+        assert torch.equal(a.u, b.u), prec
+        untouched = R.Integrator(R.Config(**kw, precision=prec), envelope=R.envelope_lens_noise(**env))
+        assert not np.array_equal(untouched.propagate()["fluence_rz"], ra["fluence_rz"])

@@ -202,3 +202,64 @@ def test_header_is_a_c_abi(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True,
                          env={**os.environ, "LD_LIBRARY_PATH": libdir})
     assert int(out.stdout) >= 100
+
+
+def test_fp16_scale_helpers_at_extremes(tmp_path):
+    """pow2f (the GEMM's power-of-two scales and the epilogue's unscale 2^-(sY + sB)) is
+    exact over the whole float range, subnormals included, 0 below 2^-149 and inf above
+    2^127; scale_exp keeps max|B| 2^se in [2^12, 2^13) (FP16 headroom) and clamps to
+    [-100, 100], returning 0 for 0, negative, inf and NaN maxima. Compiled host-only."""
+    import math
+    import shutil
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    exe = tmp_path / "scale_check"
+    src = os.path.join(ROOT, "tests", "host_checks", "scale_check.cpp")
+    r = subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-std=c++17", "-O1",
+                        "-I", os.path.join(ROOT, "rapport_amd", "csrc"), src, "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    p2, se = {}, []
+    for ln in out.splitlines():
+        k, a, b = ln.split()
+        if k == "p":
+            p2[int(a)] = float.fromhex(b)
+        else:
+            se.append((float.fromhex(a) if "0x" in a else float(a), int(b)))
+    for e, v in p2.items():
+        ref = math.inf if e > 127 else (math.ldexp(1.0, e) if e >= -149 else 0.0)
+        assert v == ref, (e, v, ref)
+    for x, e in se:
+        if not (x > 0) or math.isinf(x) or math.isnan(x):
+            assert e == 0, (x, e)
+            continue
+        assert -100 <= e <= 100, (x, e)
+        if -100 < e < 100:
+            assert 2.0 ** 12 <= x * 2.0 ** e < 2.0 ** 13, (x, e)
+        elif e == 100:
+            assert x * 2.0 ** 100 < 2.0 ** 13
+        else:
+            assert x * 2.0 ** -100 >= 2.0 ** 12
+
+
+@pytest.mark.parametrize("name", ["fp_nl_split", "fp_nl_step", "fp_record_rows",
+                                  "fp_nl_split_c64", "fp_nl_step_c64"])
+def test_nonlinear_entry_points_reject_nonpositive_rows(name):
+    """A row count <= 0 is a shape error on the host (FP_E_SHAPE), not a launch error."""
+    from rapport_amd import _lib
+    fake = ctypes.c_void_p(0x1000)
+    p = _lib.NLParams(Nr=0, Nt=512)
+    t32 = _lib.NLTablesF32(*([fake] * 9))
+    t64 = _lib.NLTables(*([fake] * 8))
+    args = {
+        "fp_nl_split": (fake, fake, None, fake, fake, ctypes.byref(p), ctypes.byref(t64), None),
+        "fp_nl_step": (fake, fake, fake, None, 1, 1, ctypes.byref(p), ctypes.byref(t64), None),
+        "fp_record_rows": (fake, fake, None, fake, ctypes.byref(p), ctypes.byref(t64), 1e-15, 0,
+                           None, None, None, None),
+        "fp_nl_split_c64": (fake, fake, None, fake, fake, ctypes.byref(p), ctypes.byref(t32),
+                            None),
+        "fp_nl_step_c64": (fake, fake, fake, None, 1, 1, ctypes.byref(p), ctypes.byref(t32), None,
+                           None),
+    }[name]
+    with pytest.raises(RuntimeError, match="Nr must be positive"):
+        _lib.call(name, *args)
