@@ -205,6 +205,8 @@ class Config:
     # EXTENSIONS (defaults reproduce the reference)
     t_window: Optional[float] = None
     write_output: bool = True
+    keldysh_monotone: bool = False    # KeldyshSiO2(monotone=...) of keldysh.py:80-148
+    keldysh_beta_den: float = 4.0     # KeldyshSiO2(beta_den=...), keldysh.py:81
 
     def __post_init__(self):                                          # config.py:202
         self.omega0 = 2 * np.pi * c / self.wavelength
@@ -326,11 +328,12 @@ def build_grids(cfg: Config) -> dict:
     mask_r = np.exp(-(rr / (0.9 * R)) ** 20)                          # :138
 
     I_knots = keldysh_knots()                                         # :140
+    kopt = dict(monotone=cfg.keldysh_monotone, beta_den=cfg.keldysh_beta_den)
     W_g = np.maximum(np.nan_to_num(
-        Keldysh(cfg.wavelength, cfg.Ui_eV, cfg.meff, cfg.n0).rate(I_knots)), 0.0)
+        Keldysh(cfg.wavelength, cfg.Ui_eV, cfg.meff, cfg.n0, **kopt).rate(I_knots)), 0.0)
     f_g = PchipInterpolator(I_knots, W_g)                             # :144-145
     W_s = np.maximum(np.nan_to_num(
-        Keldysh(cfg.wavelength, cfg.Us_eV, cfg.meff, cfg.n0).rate(I_knots)), 0.0)
+        Keldysh(cfg.wavelength, cfg.Us_eV, cfg.meff, cfg.n0, **kopt).rate(I_knots)), 0.0)
     f_s = PchipInterpolator(I_knots, W_s)                             # :149-150
     logI = np.linspace(float(np.log10(I_knots[0])), float(np.log10(I_knots[-1])), 4096)
     kel = dict(

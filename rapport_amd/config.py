@@ -9,7 +9,10 @@ passes to ``run()`` / ``Config`` mean the same thing here. Additional fields
 * ``t_window``  full width of the time window in s (reference: fixed 10 tp);
 * ``write_output``  write result.npz / params.json (reference: always);
 * ``precision``  "fp64" (drop-in arithmetic) or "mixed" (complex64 field,
-  split FP16 Hankel GEMM, FP64 plasma scan and propagator phase).
+  split FP16 Hankel GEMM, FP64 plasma scan and propagator phase);
+* ``keldysh_monotone``, ``keldysh_beta_den``  the ``monotone`` and ``beta_den`` arguments
+  of the Keldysh rate behind the photo-ionisation tables (reference: False, 4; its
+  KeldyshSiO2 takes them, D3/sim/keldysh.py:80-148, but build_grids does not pass them).
 """
 from __future__ import annotations
 
@@ -111,6 +114,8 @@ class Config:
     t_window: Optional[float] = None
     write_output: bool = True
     precision: str = "fp64"
+    keldysh_monotone: bool = False    # rate tables from KeldyshSiO2.rate(monotone=True)
+    keldysh_beta_den: float = 4.0     # KeldyshSiO2 beta_den (the reference's default 4)
 
     def __post_init__(self):
         lam = self.wavelength

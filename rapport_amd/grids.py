@@ -112,8 +112,11 @@ def build_grids(cfg) -> dict:
     mask_r = np.repeat(mask_r_1d[:, None], Nt, axis=1)
 
     I_k = intensity_knots()
-    W_g = np.maximum(np.nan_to_num(KeldyshSiO2(cfg.wavelength, cfg.Ui_eV, cfg.meff, cfg.n0).rate(I_k)), 0.0)
-    W_s = np.maximum(np.nan_to_num(KeldyshSiO2(cfg.wavelength, cfg.Us_eV, cfg.meff, cfg.n0).rate(I_k)), 0.0)
+    kopt = dict(monotone=cfg.keldysh_monotone, beta_den=cfg.keldysh_beta_den)
+    W_g = np.maximum(np.nan_to_num(
+        KeldyshSiO2(cfg.wavelength, cfg.Ui_eV, cfg.meff, cfg.n0, **kopt).rate(I_k)), 0.0)
+    W_s = np.maximum(np.nan_to_num(
+        KeldyshSiO2(cfg.wavelength, cfg.Us_eV, cfg.meff, cfg.n0, **kopt).rate(I_k)), 0.0)
     sp_g = PchipInterpolator(I_k, W_g)
     sp_s = PchipInterpolator(I_k, W_s)
     lmin, lmax = float(np.log10(I_k[0])), float(np.log10(I_k[-1]))

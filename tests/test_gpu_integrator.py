@@ -256,3 +256,60 @@ def test_in_place_edit_of_u_sets_the_field():
         assert torch.equal(a.u, b.u), prec
         untouched = R.Integrator(R.Config(**kw, precision=prec), envelope=R.envelope_lens_noise(**env))
         assert not np.array_equal(untouched.propagate()["fluence_rz"], ra["fluence_rz"])
+
+
+def _scenarios():
+    """The reference's ablation scenarios (term_ablation_study.ipynb:1242-1250): full,
+    one no_<term> per field toggle, and linear_only (all six off)."""
+    from rapport_amd import FIELD_TOGGLES
+    sc = {"full": {}}
+    for f in FIELD_TOGGLES:
+        sc["no_" + f.replace("enable_", "")] = {f: False}
+    sc["linear_only"] = {f: False for f in FIELD_TOGGLES}
+    return sc
+
+
+# 12 uJ into 30 um: every scenario ionises (rho up to 1e17-1e18 cm^-3) and stays finite; at
+# 20 uJ the reference arithmetic itself overflows without plasma absorption (dz = 10 um)
+HOT = dict(C1, nz=19, end=19 * 10e-6, energy_uJ=12.0)
+HOT_KW = dict(w0=30e-6, R_factor=20.0, save_stride=2, enable_ste=True, tau_ste=1e-12)
+
+
+@pytest.mark.parametrize("name", list(_scenarios()))
+def test_ablation_scenario_vs_oracle(name):
+    # each ablation scenario, 20 hot steps (ionisation, plasma, Raman, STE), FP64 against
+    # the oracle at 1e-9 and the mixed path against the oracle at its 1e-4 fluence bar
+    import rapport_amd as R
+    over = _scenarios()[name]
+    _announce(f"ablation scenario {name}: {over}")
+    it_p, it_o, res_p, res_o = _pair(HOT, **HOT_KW, **over)
+    assert res_o["rho_rz"].max() > 1e17 and np.isfinite(res_o["fluence_rz"]).all()
+    _compare(res_p, res_o, KEYS, 1e-9)
+    assert rel_l2(it_p.u.cpu().numpy(), it_o.u) < 1e-9
+    kw = cfg_kwargs(HOT, **HOT_KW, **over, precision="mixed")
+    res_m = R.Integrator(R.Config(**kw), envelope=R.envelope_lens_noise(**envelope_args(HOT))
+                         ).propagate()
+    e = rel_l2(res_m["fluence_rz"], res_o["fluence_rz"])
+    _announce(f"  mixed fluence rel-L2 {e:.2e}")
+    assert e <= 1e-4
+
+
+@pytest.mark.parametrize("over", [dict(keldysh_monotone=True),
+                                  dict(keldysh_beta_den=2.0),
+                                  dict(z_focus_air_um=150.0)])
+def test_host_setup_variants_vs_oracle(over):
+    # host-setup variants: the monotone Keldysh envelope and beta_den = 2 (keldysh.py:80-148)
+    # behind the rate tables, and begin from z_focus_air_um (config.py:210-215: begin =
+    # -n0 z_focus_air; end is moved with it so that dz stays 10 um), through a hot FP64 run
+    # against the oracle
+    _announce(f"host-setup variant {over}")
+    over = dict(over)
+    if "z_focus_air_um" in over:
+        over["end"] = -O.n_sellmeier(800e-9) * over["z_focus_air_um"] * 1e-6 + HOT["end"]
+    it_p, it_o, res_p, res_o = _pair(HOT, **HOT_KW, **over)
+    assert res_o["rho_rz"].max() > 1e17
+    _compare(res_p, res_o, KEYS, 1e-9)
+    if "z_focus_air_um" in over:
+        assert it_p.cfg.begin == it_o.cfg.begin == -it_o.cfg.n0 * 150.0e-6
+        assert abs(it_o.cfg.dz - 10e-6) < 1e-12
+        assert np.allclose(res_p["z"], res_o["z"]) and res_p["z"][0] < 0
