@@ -41,6 +41,8 @@ class AdjointEngine:
             raise ValueError("AdjointEngine needs a mixed-precision Integrator (precision='mixed')")
         if integ.cfg.enable_ste:
             raise ValueError("the adjoint covers STE off")
+        if integ._mx.Np != integ.Nr:
+            raise ValueError(f"the adjoint needs Nr % 128 == 0 (Nr = {integ.Nr})")
         self.it = integ
         self.mx = integ._mx
         self.ckpt = ckpt
@@ -270,7 +272,7 @@ def forward_fluence(integ, u0, n):
     for _ in range(n):
         integ._pipeline_step(integ.dz)
     if integ._mx is not None:
-        s = integ._mx.spec
+        s = integ._mx.spec_rows
         dt_invE2 = float(integ.g["dt"] * integ.g["invE2"])
         return (s.real.double() ** 2 + s.imag.double() ** 2).sum(dim=1) * (dt_invE2 / integ.Nt)
     s = integ._spec

@@ -90,6 +90,8 @@ class BatchedIntegrator:
     def _fill(self, d, m: Integrator):
         mx, t = m._mx, m.t
         mt = mx.m
+        if mx.Np != m.Nr:
+            raise ValueError(f"the batched form needs Nr % 128 == 0 (an instance has Nr = {m.Nr})")
         phase = mt.phase_half(m.dz)
         nlp = m.nl.params(m.dz)
         self._keep += [phase, nlp]

@@ -31,12 +31,26 @@ def _scale_exp(mx):
     return int(13 - e)
 
 
+def padded_rows(Nr):
+    """Row count of the mixed path's GEMM operands: Nr rounded up to a multiple of 128."""
+    return -(-int(Nr) // 128) * 128
+
+
 def split_hankel_fp16(Y):
-    """Split Y (float64, M x K) into FP16 hi/lo at scale 2^s, in the GEMM's fragment
-    order [K/32][M/16][4][16][8]: element j of lane (kg, r) is Y[16 mb + r][32 ks + 8 kg + j]."""
+    """Split Y (float64, Nr x Nr) into FP16 hi/lo at scale 2^s, in the GEMM's fragment
+    order [K/32][M/16][4][16][8]: element j of lane (kg, r) is Y[16 mb + r][32 ks + 8 kg + j].
+    Y is first zero-padded to Np x Np, Np = Nr rounded up to a multiple of 128: the padded
+    rows give zero output rows and the padded columns multiply the zero rows past Nr of the
+    field, so the product's first Nr rows are those of the unpadded Y."""
+    Nr = Y.shape[0]
+    if Y.shape != (Nr, Nr):
+        raise ValueError(f"Hankel matrix must be square, got {Y.shape}")
+    Np = padded_rows(Nr)
+    if Np != Nr:
+        Yp = np.zeros((Np, Np))
+        Yp[:Nr, :Nr] = Y
+        Y = Yp
     M, K = Y.shape
-    if M % 128 or K % 32:
-        raise ValueError(f"split-FP16 Hankel GEMM needs Nr % 128 == 0, got {M}")
     se = _scale_exp(float(np.abs(Y).max()))
     Ys = Y * 2.0 ** se
     Yh = Ys.astype(np.float16)
@@ -71,10 +85,13 @@ class MixedTables:
         self.inv_h = f32(1.0 / h)
         self.pcoef_s = f32(cs.T)                  # (n - 1) x 4, one float4 per interval
         yh, yl, se = split_hankel_fp16(g["Y"])
+        self.Np = padded_rows(self.Nr)
         self.Yh = torch.from_numpy(yh).to(dev)
         self.Yl = torch.from_numpy(yl).to(dev)
         self.y_scale_exp = se
-        self.mask_r = f32(g["mask_r_1d"])
+        mr = np.zeros(self.Np)
+        mr[:self.Nr] = g["mask_r_1d"]
+        self.mask_r = f32(mr)
         self.Rk, self.iRk = float(t64.Rk), float(t64.iRk)
         self._phase = {}
         self.tables = _lib.NLTablesF32(ptr(self.tw), ptr(self.mA), ptr(self.mB), ptr(self.mC),
@@ -90,13 +107,19 @@ class MixedTables:
             rho2 = (g["rholist"] ** 2)[:, None]
             ph = (g["delta_k"][None, :] - rho2 / (2 * g["komega"]) * g["inv_U_op"][None, :]) * dz / 2
             ph = np.remainder(ph, 2 * np.pi)
-            v = (np.exp(1j * ph) * g["spec_mask"][None, :]).astype(np.complex64)
+            v = np.zeros((self.Np, self.Nt), np.complex64)      # padded rows: 0
+            v[:self.Nr] = np.exp(1j * ph) * g["spec_mask"][None, :]
             self._phase[key] = torch.from_numpy(v).to(self.t64.device)
         return self._phase[key]
 
 
 class MixedEngine:
-    """Spectral-domain complex64 state and the mixed z-step kernel sequence."""
+    """Spectral-domain complex64 state and the mixed z-step kernel sequence.
+
+    The state and work buffers hold Np = Nr rounded up to a multiple of 128 rows (the
+    split-FP16 GEMM's tile height); rows Nr..Np-1 stay zero (the padded Hankel matrix
+    writes zeros there), and the row kernels (carrier scan, nonlinear step, record) run
+    on the first Nr rows only. ``spec_rows`` is the Nr x Nt view of the state."""
 
     def __init__(self, t64, cfg, g, u0_c128):
         self.cfg = cfg
@@ -104,7 +127,9 @@ class MixedEngine:
         self.m = MixedTables(t64, cfg, g)
         dev = t64.device
         self.Nr, self.Nt = t64.Nr, t64.Nt
-        self.spec = torch.zeros(self.Nr, self.Nt, dtype=torch.complex64, device=dev)
+        self.Np = self.m.Np
+        self.spec = torch.zeros(self.Np, self.Nt, dtype=torch.complex64, device=dev)
+        self.spec_rows = self.spec[:self.Nr]
         self.w0 = torch.zeros_like(self.spec)
         self.w1 = torch.zeros_like(self.spec)
         self.slots = torch.zeros(4, dtype=torch.int32, device=dev)
@@ -119,10 +144,10 @@ class MixedEngine:
         call("fp_fft_rows_c64", ptr(uc), ptr(self.spec), self.Nr, self.Nt, 0, ptr(self.m.tw),
              stream_ptr())
         self.slots.zero_()
-        call("fp_absmax_f32", ptr(self.spec), 2 * self.Nr * self.Nt, self._slot(0), stream_ptr())
+        call("fp_absmax_f32", ptr(self.spec), 2 * self.Np * self.Nt, self._slot(0), stream_ptr())
 
     def time_c64(self, out=None):
-        out = torch.empty_like(self.spec) if out is None else out
+        out = torch.empty_like(self.spec_rows) if out is None else out
         call("fp_fft_rows_c64", ptr(self.spec), ptr(out), self.Nr, self.Nt, 1, ptr(self.m.tw),
              stream_ptr())
         return out
@@ -140,7 +165,7 @@ class MixedEngine:
         m, st = self.m, stream_ptr()
         if clear_slots:
             self.slots[1:].zero_()
-        Nr, Nt = self.Nr, self.Nt
+        Nr, Nt = self.Np, self.Nt          # the GEMMs run over the padded rows
         ph = m.phase_half(dz)
         g = lambda B, C, a, rs, phase, bin_, cout, reset: call(  # noqa: E731
             "fp_hankel_gemm_mixed", ptr(m.Yh), ptr(m.Yl), m.y_scale_exp, ptr(B), ptr(C), Nr, Nt,

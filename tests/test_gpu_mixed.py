@@ -120,3 +120,48 @@ def test_mixed_hot_run_vs_fp64():
     e_u = rel_l2(im.u.cpu().numpy(), ip.u.cpu().numpy())
     print(f"[mixed hot] rel-L2 fluence {e_f:.2e} field {e_u:.2e} rho {e_r:.2e}")
     assert e_f < 1e-4 and e_r < 1e-3 and e_u < 1e-3
+
+
+def test_ragged_radial_grid_n201_vs_oracle():
+    # Nr = 200 (N = 201) is not a multiple of the GEMM's 128-row tile: the mixed engine pads
+    # its operands to 256 rows with zeros. Hot 40 steps: the FP64 path against the oracle at
+    # 1e-9, the mixed path against both at its 1e-4 fluence bar; padded rows stay exactly 0.
+    import rapport_amd as R
+    _announce("ragged radial grid N=201: fp64 vs oracle, mixed vs fp64 / oracle, 40 steps")
+    hot = dict(C1, N=201, nz=39, end=39 * 10e-6, energy_uJ=6.0)   # 8-12 uJ overflow (reference too)
+    over = dict(w0=30e-6, R_factor=20.0, save_stride=4)
+    env = envelope_args(hot)
+    kw = cfg_kwargs(hot, **over)
+    ito = O.Integrator(O.Config(**kw), envelope=O.envelope_lens_noise(**env))
+    ro = ito.propagate()
+    out = _runs(hot, None, 4, **{k: v for k, v in over.items() if k != "save_stride"})
+    (ip, rp), (im, rm) = out["fp64"], out["mixed"]
+    assert ip.Nr == 200 and im._mx.Np == 256
+    assert ro["rho_rz"].max() > 1e17 and np.isfinite(ito.u).all()
+    for key in ("fluence_rz", "rho_rz", "Imax_z"):
+        assert rel_l2(rp[key], ro[key]) < 1e-9, key
+    assert rel_l2(ip.u.cpu().numpy(), ito.u) < 1e-9
+    e_f = rel_l2(rm["fluence_rz"], ro["fluence_rz"])
+    e_r = rel_l2(rm["rho_rz"], ro["rho_rz"])
+    e_u = rel_l2(im.u.cpu().numpy(), ito.u)
+    _announce(f"  mixed vs oracle: fluence {e_f:.2e} field {e_u:.2e} rho {e_r:.2e}")
+    assert e_f < 1e-4 and e_r < 1e-3 and e_u < 1e-3
+    pad = im._mx.spec[im.Nr:]
+    assert pad.numel() > 0 and not bool(pad.abs().max() > 0)
+
+
+def test_reference_production_grid_n3001_mixed_vs_fp64():
+    # the reference's production radial grid (N = 3001, i.e. 3000 rows, padded to 3072 in
+    # the mixed engine; filament_visualization_1030nm.ipynb:169) at Nt = 1024, 6 hot steps
+    import rapport_amd as R
+    _announce("reference radial grid N=3001 (3000 rows) x Nt 1024: mixed vs fp64, 6 steps")
+    case = dict(C1, N=3001, Nt=1024, nz=5, end=5 * 5e-6, energy_uJ=6.0)
+    out = _runs(case, None, 1, w0=30e-6, R_factor=60.0)
+    (ip, rp), (im, rm) = out["fp64"], out["mixed"]
+    assert im.Nr == 3000 and im._mx.Np == 3072
+    e_f = rel_l2(rm["fluence_rz"], rp["fluence_rz"])
+    e_r = rel_l2(rm["rho_rz"], rp["rho_rz"])
+    e_u = rel_l2(im.u.cpu().numpy(), ip.u.cpu().numpy())
+    _announce(f"  fluence {e_f:.2e} field {e_u:.2e} rho {e_r:.2e}, max rho {rp['rho_rz'].max():.2e}")
+    assert rp["rho_rz"].max() > 1e15 and bool(torch.isfinite(ip.u).all())
+    assert e_f < 1e-4 and e_r < 1e-3 and e_u < 1e-3

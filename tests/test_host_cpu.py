@@ -165,8 +165,8 @@ def test_record_history_rejects_null_spec():
     (200, 512, "Nr % 128"), (0, 512, "Nr % 128"), (128, 100, "2 Nt % 128"),
 ])
 def test_mixed_gemm_rejects_ragged_shapes(Nr, Nt, msg):
-    """The split-FP16 GEMM tiles the radius by 128: a ragged radial grid is refused on the
-    host, by the C entry point and by the operand split, before any launch."""
+    """The split-FP16 GEMM entry point tiles the radius by 128: a ragged operand is refused
+    on the host before any launch (the mixed engine pads its operands, see below)."""
     from rapport_amd import _lib
     from rapport_amd.mixed import split_hankel_fp16
     a = 0x1000
@@ -176,9 +176,26 @@ def test_mixed_gemm_rejects_ragged_shapes(Nr, Nt, msg):
     with pytest.raises(RuntimeError, match="alias"):
         _lib.call("fp_hankel_gemm_mixed", a, a, 0, a, a, 128, 512, ctypes.c_float(1.0),
                   None, None, a, None, None, None)
-    if Nr > 0 and Nr % 128:
-        with pytest.raises(ValueError, match="Nr % 128"):
-            split_hankel_fp16(np.eye(Nr))
+
+
+@pytest.mark.parametrize("Nr", [128, 200, 300])
+def test_split_hankel_pads_to_the_tile_height(Nr):
+    """split_hankel_fp16 zero-pads Y to Np = 128 ceil(Nr / 128): un-fragmented, hi + lo is
+    Y 2^s to FP16-pair precision in the top-left Nr x Nr block and exactly 0 elsewhere."""
+    from rapport_amd.mixed import padded_rows, split_hankel_fp16
+    rng = np.random.default_rng(Nr)
+    Y = rng.standard_normal((Nr, Nr)) * 1e-3
+    yh, yl, se = split_hankel_fp16(Y)
+    Np = padded_rows(Nr)
+    assert Np % 128 == 0 and Nr <= Np < Nr + 128
+
+    def unfrag(F):
+        return F.transpose(1, 3, 0, 2, 4).reshape(Np, Np).astype(np.float64)
+    full = (unfrag(yh) + unfrag(yl)) * 2.0 ** -se
+    assert np.abs(full[:Nr, :Nr] - Y).max() <= 2.0 ** -21 * np.abs(Y).max()
+    assert not full[Nr:].any() and not full[:, Nr:].any()
+    with pytest.raises(ValueError, match="square"):
+        split_hankel_fp16(np.zeros((Nr, Nr + 1)))
 
 
 def test_header_is_a_c_abi(tmp_path):
