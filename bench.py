@@ -193,6 +193,102 @@ def accuracy(acc_steps, full_steps=0, cond_eps=1e-8):
     return out
 
 
+def _hybrid_step(it, stages):
+    """One propagate() iteration of a mixed-precision Integrator with the named stages run
+    in FP64 instead (on the complex64 state upcast to complex128, the result rounded back
+    to complex64): "plasma" (inverse transform + carrier scan), "gemm" (the four Hankel
+    GEMMs, FP64 MFMA with the FP64 half-step phase), "nl" (the nonlinear half of the Strang
+    step). Stages not named run the mixed kernels. Used to find which stage's rounding
+    drives the difference between the mixed and the FP64 runs."""
+    import ctypes
+    import torch
+    from rapport_amd._lib import call, ptr, stream_ptr
+    mx, m, t = it._mx, it._mx.m, it.t
+    Nr, Nt, Np, dz = it.Nr, it.Nt, mx.Np, it.dz
+    st = stream_ptr()
+    c128 = torch.complex128
+    if "plasma" in stages:
+        s = mx.spec_rows.to(c128)
+        call("fp_plasma_scan_spec", ptr(s), ptr(it.rho), None, None, ctypes.byref(t.plasma_params),
+             ptr(t.W_lut), ptr(t.Ws_lut), ptr(t.tw), ptr(t.status), st)
+    else:
+        mx.plasma(it.rho, it.rho_s)
+    ph32, ph64 = m.phase_half(dz), t.phase_half(dz)
+
+    def gemm(B, C, alpha32, alpha64, rs32, rs64, with_phase):
+        if "gemm" in stages:
+            b = B[:Nr].to(c128)
+            c = torch.empty_like(b)
+            call("fp_hankel_gemm", ptr(t.Yt), ptr(b), ptr(c), Nr, Nt, alpha64, ptr(rs64),
+                 ptr(ph64) if with_phase else None, st)
+            C[:Nr].copy_(c)
+        else:
+            mx.slots.zero_()
+            call("fp_absmax_f32", ptr(B), 2 * Np * Nt, mx._slot(0), st)
+            call("fp_hankel_gemm_mixed", ptr(m.Yh), ptr(m.Yl), m.y_scale_exp, ptr(B), ptr(C), Np,
+                 Nt, ctypes.c_float(alpha32), ptr(rs32), ptr(ph32) if with_phase else None,
+                 mx._slot(0), None, None, st)
+    gemm(mx.spec, mx.w1, m.Rk, t.Rk, None, None, True)
+    gemm(mx.w1, mx.w0, m.iRk, t.iRk, None, None, False)
+    if "nl" in stages:
+        w = mx.w0[:Nr].to(c128)
+        call("fp_nl_step", ptr(w), ptr(w), ptr(it.rho), None, 1, 1,
+             ctypes.byref(it.nl.params(dz)), ctypes.byref(t.tables), st)
+        mx.w0[:Nr].copy_(w)
+    else:
+        call("fp_nl_step_c64", ptr(mx.w0), ptr(mx.w0), ptr(it.rho), None, 1, 1,
+             ctypes.byref(it.nl.params(dz)), ctypes.byref(m.tables), None, st)
+    gemm(mx.w0, mx.w1, m.Rk, t.Rk, None, None, True)
+    gemm(mx.w1, mx.spec, m.iRk, t.iRk, m.mask_r, t.mask_r, False)
+
+
+def accuracy_sources(steps, every=40):
+    """Where the mixed path's difference from FP64 comes from, over `steps` steps of the
+    1024 x 4096 workload (fluence recorded every `every` steps, rel-L2 per record against
+    the FP64 run):
+    * floor: the FP64 pipeline with its state rounded to complex64 after every step -- the
+      error every method that stores the field in complex64 starts from;
+    * mixed with one stage at a time in FP64 (plasma, gemm, nl; see _hybrid_step), and
+      with all three (then only the complex64 storage between the stages is left)."""
+    import numpy as np
+    import torch
+
+    def up(x):
+        return np.asarray(x).astype(np.float64)
+
+    def curve(res, ref):
+        return [float(np.linalg.norm(up(res[j]) - up(ref[j])) / np.linalg.norm(up(ref[j])))
+                for j in range(ref.shape[0])]
+    it = _integrator("fp64", save_stride=every)
+    ref = it.propagate(steps)["fluence_rz"]
+    del it
+    out = {}
+    it = _integrator("fp64", save_stride=every)
+    for i in range(steps):
+        it._pipeline_step(it.dz)
+        it._spec.copy_(it._spec.to(torch.complex64).to(torch.complex128))
+        if i % every == 0:
+            it._record(i * it.dz)
+    out["floor_fp64_c64_state"] = curve(it._results()["fluence_rz"], ref)
+    del it
+    variants = {"mixed": (), "plasma_fp64": ("plasma",), "gemm_fp64": ("gemm",),
+                "nl_fp64": ("nl",), "all_stages_fp64": ("plasma", "gemm", "nl")}
+    for name, stages in variants.items():
+        it = _integrator("mixed", save_stride=every)
+        for i in range(steps):
+            _hybrid_step(it, stages)
+            if i % every == 0:
+                it._record(i * it.dz)
+        out[name] = curve(it._results()["fluence_rz"], ref)
+        del it
+    torch.cuda.synchronize()
+    pick = range(0, len(ref), max(1, 200 // every))
+    return dict(steps=steps, every=every,
+                final={k: v[-1] for k, v in out.items()},
+                rms_over_run={k: float(np.sqrt(np.mean(np.square(v)))) for k, v in out.items()},
+                per_step_fluence={k: {str(every * j): v[j] for j in pick} for k, v in out.items()})
+
+
 def run_batched(n_inst=64, steps=20, seed=0, tune=True):
     """Batched sweep of n_inst instances (Nr in {256, 512, 768, 1024}, Nt in {1024, 2048,
     4096}, 0.5-20 uJ log-uniform, 40-200 fs, w0 50-150 um, 1 % noise, dz 10 um), tuned,
@@ -314,20 +410,25 @@ def run_adjoint(n_fd=4, h=1e-4, P=16, seed=5, fit_iters=20):
                                                            rel_l2=float(np.linalg.norm(ga - fd) / np.linalg.norm(fd))))
 
 
-def run_cpu_baseline(n_steps=2):
-    """Oracle (NumPy FP64 restatement of the reference) on the host: 1 warm-up
-    step then n_steps timed steps of the same workload."""
+def run_cpu_baseline(n_steps=5, repeats=3):
+    """Oracle (NumPy FP64 restatement of the reference) on the host: one warm-up step,
+    then `repeats` runs of n_steps timed steps of the same workload; the median run."""
+    import statistics
     from oracle import reference_numpy as O
     kw = dict(WORKLOAD, save_stride=10 ** 9)
     it = O.Integrator(O.Config(**kw), envelope=O.envelope_lens_noise(**ENVELOPE))
     it.advance(1)
-    t0 = time.perf_counter()
-    for i in range(n_steps):
-        it.advance(2 + i)
-    dt = time.perf_counter() - t0
-    return dict(value=n_steps / dt, unit="z-steps/s", cores=_cores(), kind="port",
-                sample=f"{n_steps} z-steps (after 1 warm-up) of the 1024x4096 workload, NumPy/SciPy "
-                       f"FP64 oracle, BLAS/FFT threads = {_cores()}")
+    rates, i = [], 2
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            it.advance(i)
+            i += 1
+        rates.append(n_steps / (time.perf_counter() - t0))
+    return dict(value=statistics.median(rates), unit="z-steps/s", cores=_cores(), kind="port",
+                runs=rates,
+                sample=f"median of {repeats} runs of {n_steps} z-steps (after 1 warm-up) of the "
+                       f"1024x4096 workload, NumPy/SciPy FP64 oracle, BLAS threads = {_cores()}")
 
 
 def copy_calibration(nbytes=256 * 2 ** 20, reps=5):
@@ -415,6 +516,8 @@ def main():
     ap.add_argument("--full-steps", type=int, default=4000,
                     help="continue the accuracy runs to this many steps (0: gate window only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sources-steps", type=int, default=4000,
+                    help="steps of the error-source runs (floor and stage swaps; 0: skip)")
     ap.add_argument("--no-small-accuracy", action="store_true",
                     help="skip the mixed-vs-FP64 check at 128 x 512")
     ap.add_argument("--batched", type=int, default=64,
@@ -444,7 +547,8 @@ def main():
     r = runs[head]
     roof, ng = _roofline(head, r)
     out = dict(
-        metric="z-steps/s at Nr x Nt = 1024 x 4096 (mixed vs FP64) at fluence rel-L2 <= 1e-4",
+        metric="z-steps/s at Nr x Nt = 1024 x 4096 (mixed vs FP64) at fluence rel-L2 <= 1e-4 "
+               f"over {args.acc_steps} steps",
         value=r["steps_per_s"], unit="z-steps/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=r["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype=("c64 field / fp16x3 MFMA fp32-accum / f64 plasma" if head == "mixed" else "f64"),
@@ -470,6 +574,8 @@ def main():
                                         and not args.no_small_accuracy else None),
         plasma_scan_capped_rows=sum(x["capped_rows"] for x in runs.values()),
     )
+    if args.precision == "both" and args.sources_steps > 0:
+        out["accuracy_sources"] = accuracy_sources(args.sources_steps)
     if args.batched > 0:
         out["batched"] = run_batched(args.batched, args.batched_steps)
     if not args.no_adjoint:

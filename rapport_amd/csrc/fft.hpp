@@ -12,7 +12,9 @@
 // Conventions are numpy's: forward X[k] = sum_n x[n] exp(-2 pi i k n / N),
 // inverse has +i and is NOT scaled here (callers fold the 1/N).
 //
-// Twiddles come from a per-N table tw[m] = exp(-2 pi i m / N), m < N.
+// Twiddles come from a per-N table tw[m] = exp(-2 pi i m / N), m < N. The exact float
+// variant (EXACT, E = 4 or 8 samples per thread) reads compact per-pass tables stored after
+// tw[2 N] instead (exact_table_offset).
 #pragma once
 #include "common.hpp"
 
@@ -46,16 +48,43 @@ template <typename T> FP_DEV T s16(int k) {
   return T(tab[k & 7]);
 }
 
-// x * exp(-+2 pi i idx/16), idx in [0, 8)
-template <typename T, bool INV> FP_DEV cpx<T> rot16(cpx<T> x, int idx) {
+// The float roundings of cos / sin(2 pi k / 16) all lie inside the unit circle
+// (|(c, s)| - 1 = -1.7e-8 for k = 2, 6 and -2.9e-8 for odd k), so every rotation by them
+// shrinks its operand: a coherent loss of about 6e-8 in amplitude per float transform.
+// Their residuals (cos - float(cos), likewise sin) let the exact variant below form each
+// product to within the rounding of its result.
+template <typename T> FP_DEV T c16lo(int k) {
+  constexpr double tab[8] = {0.0, 0.92387953251128675613, 0.70710678118654752440,
+                             0.38268343236508977173, 0.0, -0.38268343236508977173,
+                             -0.70710678118654752440, -0.92387953251128675613};
+  return T(tab[k & 7] - (double)(T)tab[k & 7]);
+}
+template <typename T> FP_DEV T s16lo(int k) {
+  constexpr double tab[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440,
+                             0.92387953251128675613, 1.0, 0.92387953251128675613,
+                             0.70710678118654752440, 0.38268343236508977173};
+  return T(tab[k & 7] - (double)(T)tab[k & 7]);
+}
+
+// x * exp(-+2 pi i idx/16), idx in [0, 8). EXACT (float only): products with the
+// constant split in a rounded part and its residual, fused, so the rotation has no
+// magnitude bias.
+template <typename T, bool INV, bool EXACT = false> FP_DEV cpx<T> rot16(cpx<T> x, int idx) {
   if (idx == 0) return x;
   if (idx == 4) return mul_mi<T, INV>(x);
   const T c = c16<T>(idx), s = INV ? s16<T>(idx) : -s16<T>(idx);
+  if constexpr (EXACT && sizeof(T) == 4) {
+    const T cl = c16lo<T>(idx), sl = INV ? s16lo<T>(idx) : -s16lo<T>(idx);
+    const T re_lo = __builtin_fmaf(x.x, cl, -x.y * sl);
+    const T im_lo = __builtin_fmaf(x.x, sl, x.y * cl);
+    return mk<T>(__builtin_fmaf(x.x, c, __builtin_fmaf(-x.y, s, re_lo)),
+                 __builtin_fmaf(x.x, s, __builtin_fmaf(x.y, c, im_lo)));
+  }
   return mk<T>(x.x * c - x.y * s, x.x * s + x.y * c);
 }
 
 // In-register DFT of size R (R in {2,4,8,16}), natural order in and out.
-template <typename T, int R, bool INV> FP_DEV void dft(cpx<T> (&a)[R]) {
+template <typename T, int R, bool INV, bool EXACT = false> FP_DEV void dft(cpx<T> (&a)[R]) {
   constexpr int LB = ilog2(R);
 #pragma unroll
   for (int len = R; len >= 2; len >>= 1) {
@@ -66,7 +95,7 @@ template <typename T, int R, bool INV> FP_DEV void dft(cpx<T> (&a)[R]) {
       for (int m = 0; m < half; ++m) {
         cpx<T> x = a[st + m], y = a[st + m + half];
         a[st + m] = x + y;
-        a[st + m + half] = rot16<T, INV>(x - y, m * (R / len) * (16 / R));
+        a[st + m + half] = rot16<T, INV, EXACT>(x - y, m * (R / len) * (16 / R));
       }
     }
   }
@@ -77,7 +106,28 @@ template <typename T, int R, bool INV> FP_DEV void dft(cpx<T> (&a)[R]) {
   for (int r = 0; r < R; ++r) a[r] = b[r];
 }
 
-template <typename T, int N, int TPB, bool INV, int Ns>
+// Compact twiddle tables of the exact float transform, stored after the 2 N entries of
+// the twiddle table: for a row held E samples per thread the passes are radix
+// R = min(E, N / Ns) at Ns = 1, R, R R', ...; each twiddled pass (Ns > 1) stores, for k < Ns
+// and r = 1 .. R-1, the float4 (w^r rounded, residual), w = exp(-2 pi i k N / (Ns R) / N).
+// The table for E = 8 comes first, then the one for E = 4.
+constexpr int exact_pass_entries(int N, int E, int Ns) {
+  return Ns >= N ? 0
+                 : (Ns > 1 ? Ns * ((N / Ns >= E ? E : N / Ns) - 1) : 0) +
+                       exact_pass_entries(N, E, Ns * (N / Ns >= E ? E : N / Ns));
+}
+template <int N, int E, int NsTarget>
+constexpr int exact_table_offset() {
+  int off = E == 4 ? exact_pass_entries(N, 8, 1) : 0;
+  for (int ns = 1; ns < NsTarget;) {
+    const int r = N / ns >= E ? E : N / ns;
+    if (ns > 1) off += ns * (r - 1);
+    ns *= r;
+  }
+  return off;
+}
+
+template <typename T, int N, int TPB, bool INV, int Ns, bool EXACT = false>
 FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw, int tid) {
   constexpr int E = N / TPB;
   constexpr int REM = N / Ns;
@@ -93,7 +143,24 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
     if (Ns > 1) {
-      if (sizeof(T) == 8) {
+      if (EXACT && sizeof(T) == 4) {
+        // every power from the pass's compact table (exact_table_offset): one float4
+        // (rounded w^r, residual) per power, the powers of a butterfly contiguous, so the
+        // loads of a wave are contiguous; products fused with the residual: no rounding
+        // of the twiddles is raised to a power, and the transform has no coherent gain
+        const float4* pt = reinterpret_cast<const float4*>(tw + 2 * N) +
+                           exact_table_offset<N, E, Ns>() + k * (R - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const float4 e = pt[r - 1];
+          const T hy = INV ? -e.y : e.y, ly = INV ? -e.w : e.w;
+          const cpx<T> x = a[r];
+          const T re_lo = __builtin_fmaf(x.x, e.z, -x.y * ly);
+          const T im_lo = __builtin_fmaf(x.x, ly, x.y * e.z);
+          a[r] = mk<T>(__builtin_fmaf(x.x, e.x, __builtin_fmaf(-x.y, hy, re_lo)),
+                       __builtin_fmaf(x.x, hy, __builtin_fmaf(x.y, e.x, im_lo)));
+        }
+      } else if (sizeof(T) == 8) {
         // FP64: powers of one table twiddle (rounding growth ~R ulp is below FP64 needs)
         cpx<T> w1 = tw[k * STRIDE];
         if (INV) w1 = conjx(w1);
@@ -132,7 +199,7 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
         for (int r = 1; r < R; ++r) a[r] = a[r] * (INV ? conjx(w[r]) : w[r]);
       }
     }
-    dft<T, R, INV>(a);
+    dft<T, R, INV, EXACT>(a);
     if (LAST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q + r * Q] = a[r];
@@ -155,21 +222,22 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
       for (int s = 0; s < E; ++s) v[s] = lds[lds_pad(tid + s * TPB)];
     }
     __syncthreads();
-    passes<T, N, TPB, INV, Ns * R>(v, lds, tw, tid);
+    passes<T, N, TPB, INV, Ns * R, EXACT>(v, lds, tw, tid);
   }
 }
 
 // Full-row FFT in the stride layout. `lds` must hold LdsSize<N>::value entries
 // and must not be in use by other threads of the workgroup when called.
-template <typename T, int N, int TPB, bool INV>
+template <typename T, int N, int TPB, bool INV, bool EXACT = false>
 FP_DEV void row_fft(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw) {
   static_assert(N % TPB == 0, "N must be a multiple of TPB");
+  static_assert(!EXACT || N / TPB == 4 || N / TPB == 8, "exact transform: 4 or 8 samples per thread");
   // Opaque copy of the thread index: keeps the compiler from hoisting this
   // transform's LDS / twiddle addresses out of enclosing loops and holding
   // them in registers across the whole kernel.
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
-  passes<T, N, TPB, INV, 1>(v, lds, tw, tid);
+  passes<T, N, TPB, INV, 1, EXACT>(v, lds, tw, tid);
 }
 
 }  // namespace fpfft

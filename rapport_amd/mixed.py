@@ -61,6 +61,58 @@ def split_hankel_fp16(Y):
     return frag(Yh), frag(Yl), se
 
 
+def _exact_pass_table(N, E):
+    """Compact twiddles of the exact float transform for rows of E samples per thread
+    (fft.hpp, exact_table_offset): per twiddled pass (radix R = min(E, N / Ns) at
+    Ns = 1, R, ...; Ns > 1), for k < Ns and r = 1 .. R-1, the pair (w^r rounded to
+    complex64, residual), w = exp(-2 pi i k / (Ns R)), as 4 floats."""
+    out, ns = [], 1
+    while ns < N:
+        R = min(E, N // ns)
+        if ns > 1:
+            k = np.arange(ns)[:, None]
+            r = np.arange(1, R)[None, :]
+            z = np.exp(-2j * np.pi * k * r / (ns * R))
+            hi = z.astype(np.complex64)
+            lo = (z - hi.astype(np.complex128)).astype(np.complex64)
+            out.append(np.stack([hi.real, hi.imag, lo.real, lo.imag], axis=-1).reshape(-1, 4))
+        ns *= R
+    return np.concatenate(out).astype(np.float32) if out else np.zeros((0, 4), np.float32)
+
+
+def exact_twiddles(N):
+    """Twiddle table of the complex64 kernels: tw[m] = exp(-2 pi i m / N) rounded (m < N),
+    their residuals tw[N + m] = exp(-2 pi i m / N) - tw[m], then the compact per-pass tables
+    of the exact transform for 8 and for 4 samples per thread (complex64 array)."""
+    z = np.exp(-2j * np.pi * np.arange(N) / N)
+    hi = z.astype(np.complex64)
+    lo = (z - hi.astype(np.complex128)).astype(np.complex64)
+    t8, t4 = _exact_pass_table(N, 8), _exact_pass_table(N, 4)
+    flat = np.concatenate([hi.view(np.float32), lo.view(np.float32), t8.ravel(), t4.ravel()])
+    return np.ascontiguousarray(flat.view(np.complex64))
+
+
+def matched_scale_pair(x, search=4096):
+    """Float32 pair (a, b) with a within `search` ulps of x and a * b as close to 1 as the
+    search finds (typically within 1e-10; plain float32(x) * float32(1 / x) is off by up to
+    6e-8). The Hankel transform pair scales its two GEMMs by R^2 / j_N and j_N / R^2: in
+    float32 their product would be a coherent gain of up to 1.3e-7 per z-step on the whole
+    field (5e-4 in amplitude after 4000 steps); a matched pair leaves only the incoherent
+    rounding of the products."""
+    x32 = np.float32(x)
+    ia = np.arange(-search, search + 1, dtype=np.int64) + int(x32.view(np.int32))
+    a = ia.astype(np.int32).view(np.float32).astype(np.float64)
+    b0 = (1.0 / a).astype(np.float32)
+    best = None
+    for k in (-1, 0, 1):
+        b = np.nextafter(b0, np.float32(np.inf if k > 0 else -np.inf)) if k else b0
+        err = np.abs(a * b.astype(np.float64) - 1.0)
+        j = int(np.argmin(err))
+        if best is None or err[j] < best[0]:
+            best = (float(err[j]), float(a[j]), float(b[j]))
+    return best[1], best[2]
+
+
 class MixedTables:
     """Float32 / FP16 device tables of the mixed path, derived from the FP64 ones."""
 
@@ -71,7 +123,7 @@ class MixedTables:
         Nt = self.Nt
         c64 = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.complex64))).to(dev)  # noqa: E731
         f32 = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32))).to(dev)     # noqa: E731
-        self.tw = c64(np.exp(-2j * np.pi * np.arange(Nt) / Nt))
+        self.tw = torch.from_numpy(exact_twiddles(Nt)).to(dev)
         T, iU = g["T_op"], g["inv_U_nl"]
         self.mA, self.mB, self.mC = f32(T * T * iU), f32(-T * iU), f32(iU)
         self.Rf = c64(g["R_f"])
@@ -92,7 +144,7 @@ class MixedTables:
         mr = np.zeros(self.Np)
         mr[:self.Nr] = g["mask_r_1d"]
         self.mask_r = f32(mr)
-        self.Rk, self.iRk = float(t64.Rk), float(t64.iRk)
+        self.Rk, self.iRk = matched_scale_pair(t64.Rk)
         self._phase = {}
         self.tables = _lib.NLTablesF32(ptr(self.tw), ptr(self.mA), ptr(self.mB), ptr(self.mC),
                                        ptr(self.Rf), ptr(self.knots), ptr(self.inv_h),
