@@ -260,6 +260,31 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
                          const float* phase, const unsigned* bmax_in, unsigned* cmax_out,
                          unsigned* reset_slot, void* stream);
 
+/* fp_hankel_gemm restricted to the columns outside [zc_lo, zc_hi) (real columns), where
+ * the caller knows the product is exactly zero (the spectral mask is 0 there): the other
+ * columns are tiled to fill the GPU once and are bitwise fp_hankel_gemm's values; columns in
+ * the range are not written, or, with zero_fill, set to 0. Falls back to fp_hankel_gemm
+ * when the range is too narrow to gain anything or Nr % 128 != 0. Replaces
+ * cp.dot(self.Y, u) (D3/sim/operators.py:48,53) in the z-step pipeline. */
+int fp_hankel_gemm_band(const double* Yt, const double* B, double* C, int Nr, int Nt,
+                        double alpha, const double* rowscale, const double* phase, int zc_lo,
+                        int zc_hi, int zero_fill, void* stream);
+
+/* fp_hankel_gemm_mixed restricted to the columns outside [zc_lo, zc_hi) (real columns,
+ * 0 <= zc_lo <= zc_hi <= 2 Nt), where the caller knows the product is exactly zero: the
+ * spectral mask is 0 there, so the G1 / G3 outputs (phase x mask) and the G2 / G4 inputs
+ * vanish. The remaining columns are tiled so that the tiles fill the GPU once; columns in
+ * the range are not written, or, with zero_fill, set to 0 (needed where a later row
+ * kernel reads the full row). Same results as fp_hankel_gemm_mixed in the columns
+ * computed. When the range is too narrow to gain anything (or Nr % 256 != 0) it is
+ * fp_hankel_gemm_mixed. Replaces cp.dot(self.Y, u) (D3/sim/operators.py:48,53) in the
+ * z-step pipeline. */
+int fp_hankel_gemm_mixed_band(const void* Yh, const void* Yl, int y_scale_exp, const float* B,
+                              float* C, int Nr, int Nt, float alpha, const float* rowscale,
+                              const float* phase, const unsigned* bmax_in, unsigned* cmax_out,
+                              unsigned* reset_slot, int zc_lo, int zc_hi, int zero_fill,
+                              void* stream);
+
 /* split() on complex64 rows (cf. fp_nl_split). */
 int fp_nl_split_c64(const float* u, const double* rho, const double* rho_s, float* rhs,
                     float* alpha, const fp_nl_params* p, const fp_nl_tables_f32* t,

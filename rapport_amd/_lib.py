@@ -93,6 +93,9 @@ _SIGS = {
     "fp_fft_rows_c64": [P, P, c_int, c_int, c_int, P, P],
     "fp_plasma_scan_spec_c64": [P, P, P, P, ctypes.POINTER(PlasmaParams), P, P, P, P, P, P],
     "fp_hankel_gemm_mixed": [P, P, c_int, P, P, c_int, c_int, ctypes.c_float, P, P, P, P, P, P],
+    "fp_hankel_gemm_band": [P, P, P, c_int, c_int, c_double, P, P, c_int, c_int, c_int, P],
+    "fp_hankel_gemm_mixed_band": [P, P, c_int, P, P, c_int, c_int, ctypes.c_float, P, P, P, P, P,
+                                  c_int, c_int, c_int, P],
     "fp_nl_split_c64": [P, P, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTablesF32), P],
     "fp_nl_step_c64": [P, P, P, P, c_int, c_int, ctypes.POINTER(NLParams),
                        ctypes.POINTER(NLTablesF32), P, P],

@@ -27,6 +27,7 @@ from . import _lib
 from ._lib import call, ptr, stream_ptr
 from .config import Config
 from .integrator import Integrator
+from .mixed import zero_columns  # noqa: F401  (re-exported: the batch descriptors use it)
 
 PASSES = dict(plasma=0, G1=1, G2=2, nl=3, G3=4, G4=5)
 KIND = dict(plasma=0, gemm=1, nl=2)
@@ -34,18 +35,6 @@ KIND = dict(plasma=0, gemm=1, nl=2)
 
 def _vp(t):
     return None if t is None else t.data_ptr()
-
-
-def zero_columns(spec_mask):
-    """Real-column range [lo, hi) of the frequencies where the spectral mask is exactly 0
-    (the longest contiguous run; (0, 0) if none): every Hankel GEMM output of the step is
-    exactly zero there (the half-step phase carries the mask)."""
-    z = np.flatnonzero(np.asarray(spec_mask) == 0.0)
-    if z.size == 0:
-        return 0, 0
-    runs = np.split(z, np.flatnonzero(np.diff(z) != 1) + 1)
-    r = max(runs, key=len)
-    return 2 * int(r[0]), 2 * (int(r[-1]) + 1)
 
 
 class BatchedIntegrator:

@@ -167,4 +167,148 @@ __global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64(const double* __restr
   }
 }
 
+// Column band of the FP64 GEMM (see the mixed GEMM's ColBand): virtual column v is real
+// column v + (v >= va ? shift : 0).
+struct Band64 {
+  int va, shift;
+  FP_DEV int map(int v) const { return v >= va ? v + shift : v; }
+};
+
+// The FP64 GEMM restricted to a column band: tiles of 128 rows x TBK 16-column blocks,
+// `slots` tiles per row panel over the band's virtual columns, two blocks per CU. The four
+// waves each take 32 rows and all TBK blocks (equal work per SIMD whatever TBK is). Same
+// operand staging, K order, phase epilogue and per-element arithmetic as hankel_gemm_f64,
+// so the computed columns are bitwise its values. Needs M % 128 == 0. Afterwards each
+// block writes zeros to its share of the real columns [zero_lo, zero_hi).
+template <int TBK>
+__global__ __launch_bounds__(NTHR, 2) void hankel_gemm_f64_band(
+    const double* __restrict__ Yt, const double* __restrict__ B, double* __restrict__ C, int M,
+    int K, int Ncols, double alpha, const double* __restrict__ rowscale,
+    const double* __restrict__ phase, int slots, Band64 band, int zero_lo, int zero_hi) {
+  __shared__ __attribute__((aligned(16))) double As[2][BK][BM + APAD];
+  __shared__ __attribute__((aligned(16))) double Bs[2][BK][BN + APAD];
+  constexpr int FA = 2, NCB = TBK * 16;
+  const int tilesM = M / BM;
+  const int nb = tilesM * slots;
+  int lin = blockIdx.x;
+  if (nb % 8 == 0) lin = (lin % 8) * (nb / 8) + lin / 8;
+  const int tm = lin % tilesM, sl = lin / tilesM;
+  const int m0 = tm * BM, n0 = sl * NCB;          // n0: virtual column
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave * 32;
+
+  const int ldk = tid >> 4;
+  const int ldc = (tid & 15) * 2;
+  int bcol[4];                                     // real column of each staged pair
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int c = ldc + 32 * i;
+    bcol[i] = band.map(n0 + (c < NCB ? c : NCB - 2));
+  }
+  double2 ra[4], rb[4];
+  auto gload = [&](int k0) {
+    const int k = k0 + ldk;
+    const double* pa = Yt + (size_t)k * M + m0 + ldc;
+    const double* pb = B + (size_t)k * Ncols;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (k < K) {
+        ra[i] = *reinterpret_cast<const double2*>(pa + 32 * i);
+        rb[i] = *reinterpret_cast<const double2*>(pb + bcol[i]);
+      } else {
+        ra[i] = double2{0.0, 0.0};
+        rb[i] = double2{0.0, 0.0};
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      *reinterpret_cast<double2*>(&As[buf][ldk][ldc + 32 * i]) = ra[i];
+      *reinterpret_cast<double2*>(&Bs[buf][ldk][ldc + 32 * i]) = rb[i];
+    }
+  };
+
+  d4 acc[FA][TBK];
+#pragma unroll
+  for (int a = 0; a < FA; ++a)
+#pragma unroll
+    for (int b = 0; b < TBK; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int nslab = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int li = lane & 15, lk = lane >> 4;
+
+  for (int s = 0; s < nslab; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nslab) gload((s + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double av[FA], bv[TBK];
+#pragma unroll
+      for (int a = 0; a < FA; ++a) av[a] = As[cur][kk + lk][wm + a * 16 + li];
+#pragma unroll
+      for (int b = 0; b < TBK; ++b) bv[b] = Bs[cur][kk + lk][b * 16 + li];
+#pragma unroll
+      for (int a = 0; a < FA; ++a)
+#pragma unroll
+        for (int b = 0; b < TBK; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    if (s + 1 < nslab) {
+      sstore(cur ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // epilogue: the phase values of one 16-row block at a time (wider tiles leave no
+  // registers to prefetch them under the last slab, as hankel_gemm_f64 does)
+#pragma unroll
+  for (int a = 0; a < FA; ++a) {
+    double php[4][TBK];
+    if (phase) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm + a * 16 + lk + 4 * v;
+#pragma unroll
+        for (int b = 0; b < TBK; ++b)
+          php[v][b] = phase[(size_t)m * Ncols + band.map(n0 + b * 16 + li)];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + wm + a * 16 + lk + 4 * v;
+      const double rs = rowscale ? alpha * rowscale[m] : alpha;
+#pragma unroll
+      for (int b = 0; b < TBK; ++b) {
+        const int n = band.map(n0 + b * 16 + li);
+        double val = acc[a][b][v] * rs;
+        if (phase) {
+          const double other = __shfl_xor(val, 1, 64);
+          const double own_ph = php[v][b];
+          const double other_ph = __shfl_xor(own_ph, 1, 64);
+          const double pr = (n & 1) ? other_ph : own_ph;
+          const double pi = (n & 1) ? own_ph : other_ph;
+          val = (n & 1) ? (other * pi + val * pr) : (val * pr - other * pi);
+        }
+        C[(size_t)m * Ncols + n] = val;
+      }
+    }
+  }
+  if (zero_hi > zero_lo) {
+    const int w2 = (zero_hi - zero_lo) / 2;
+    const long long total = (long long)M * w2;
+    const long long chunk = (total + gridDim.x - 1) / gridDim.x;
+    const long long end = min(total, (long long)(blockIdx.x + 1) * chunk);
+    for (long long i = (long long)blockIdx.x * chunk + threadIdx.x; i < end; i += NTHR) {
+      const int r = (int)(i / w2), c2 = (int)(i % w2);
+      *reinterpret_cast<double2*>(C + (size_t)r * Ncols + zero_lo + 2 * c2) = double2{0.0, 0.0};
+    }
+  }
+}
+
 }  // namespace fpgemm

@@ -70,23 +70,43 @@ template <int BM, int BN, int WM = 2, int WN = 2> struct Smem {
   float red[WM * WN];
 };
 
-// One BM x BN output tile (tm, tn) of C = alpha diag(rowscale) Y B (x phase).
-template <int BM, int BN, int WM = 2, int WN = 2, int NPF = 0>
+// Column band of a tile: the tile's columns are a run of a "virtual" column space that
+// skips the frequencies where the spectral mask is exactly 0 (real columns [va, va + shift)
+// of C and B are left out: virtual column v is real column v + (v >= va ? shift : 0)).
+struct ColBand {
+  int va, shift;
+  FP_DEV int map(int v) const { return v >= va ? v + shift : v; }
+};
+constexpr ColBand kNoBand = {1 << 30, 0};
+
+// One BM x BN output tile (tm, tn) of C = alpha diag(rowscale) Y B (x phase). With
+// TBK < BN / 16 the tile is TBK 16-column blocks wide (the rest of the B panel is staged
+// but not multiplied or stored); the two wave columns then take ceil(TBK / 2) and
+// floor(TBK / 2) blocks, and wave w has wave row w % WM and wave column w / WM, so each
+// SIMD (w % 4) holds one wave of each column and the SIMDs stay balanced. The tile's first
+// virtual column is n0v.
+template <int BM, int BN, int WM = 2, int WN = 2, int NPF = 0, int TBK = BN / 16>
 FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __restrict__ Yh,
                       const h8* __restrict__ Yl, int y_scale_exp, const float* __restrict__ B,
                       float* __restrict__ C, int M, int K, int Ncols, float alpha,
                       const float* __restrict__ rowscale, const float* __restrict__ phase,
-                      const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out) {
+                      const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out,
+                      ColBand band = kNoBand, int n0v = -1) {
   const int tid = threadIdx.x;
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * BM, n0 = n0v >= 0 ? n0v : tn * BN;
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int NT = WM * WN * 64;
   constexpr int FA = BM / WM / 16, FB = BN / WN / 16;   // 16-row / 16-col blocks per wave
   constexpr int CPT = NT / BN;                     // threads per B column
   constexpr int GPT = 4 / CPT;                     // k groups of 8 staged per thread
   constexpr int APT = BM * 4 / NT;                 // A fragments staged per thread
+  constexpr bool NARROW = TBK < BN / 16;
   static_assert(GPT >= 1 && APT >= 1, "tile too small for the block");
-  const int wm = (wave / WN) * (BM / WM), wn = (wave % WN) * (BN / WN);
+  static_assert(!NARROW || (WN == 2 && (TBK + 1) / 2 <= FB), "narrow tiles: two wave columns");
+  const int wm = NARROW ? (wave % WM) * (BM / WM) : (wave / WN) * (BM / WM);
+  // narrow tiles: wave column 1 starts right after wave column 0's ceil(TBK / 2) blocks
+  const int wn = NARROW ? (wave / WM) * ((TBK + 1) / 2) * 16 : (wave % WN) * (BN / WN);
+  const bool wcol0 = wn == 0;
 
   const int sb_exp = scale_exp(__uint_as_float(*bmax_in));
   const float sB = pow2f(sb_exp);
@@ -110,7 +130,7 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
   const char* Bc = reinterpret_cast<const char*>(B);
   const char* Yhc = reinterpret_cast<const char*>(Yh);
   const char* Ylc = reinterpret_cast<const char*>(Yl);
-  const unsigned bcol4 = 4u * (unsigned)(n0 + bc);
+  const unsigned bcol4 = 4u * (unsigned)band.map(n0 + (bc < TBK * 16 ? bc : TBK * 16 - 1));
   const unsigned row4 = 4u * (unsigned)Ncols;
   auto gload = [&](int ks, Stage& R) {
     const unsigned k0 = (unsigned)(ks * BK);
@@ -161,8 +181,10 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
   const int lq = lane >> 4, lr = lane & 15;
   // MFMA phase of a slab at raised wave priority, with the scheduler told to interleave the
   // B-fragment LDS reads of the next column block between the current block's MFMAs (both
-  // measured: 10 % faster at 1024 x 4096, results bitwise unchanged).
-  auto mma = [&](int cur) {
+  // measured: 10 % faster at 1024 x 4096, results bitwise unchanged). NB: column blocks
+  // this wave multiplies (FB, or its share of a narrow tile).
+  auto mma_n = [&](int cur, auto nb_c) {
+    constexpr int NB = decltype(nb_c)::value;
     __builtin_amdgcn_s_setprio(1);
     h8 ah[FA], al[FA];
 #pragma unroll
@@ -173,7 +195,7 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
     }
     // B fragments per column block: keeps the live registers at A + one B pair
 #pragma unroll
-    for (int b = 0; b < FB; ++b) {
+    for (int b = 0; b < NB; ++b) {
       const int idx = (((wn >> 4) + b) * 4 + lq) * 16 + lr;
       const h8 bh = sm.Bh[cur][idx];
       const h8 bl = sm.Bl[cur][idx];
@@ -188,13 +210,22 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
     // half of its 3 FA MFMAs, the next block's two B reads, the other half
     __builtin_amdgcn_sched_group_barrier(0x100, 2 * FA + 2, 0);
 #pragma unroll
-    for (int b = 0; b < FB; ++b) {
+    for (int b = 0; b < NB; ++b) {
       __builtin_amdgcn_sched_group_barrier(0x008, 3 * FA / 2, 0);
-      if (b + 1 < FB) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+      if (b + 1 < NB) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
       __builtin_amdgcn_sched_group_barrier(0x008, 3 * FA / 2, 0);
     }
     __builtin_amdgcn_s_setprio(0);
   };
+  auto mma = [&](int cur) {
+    if constexpr (NARROW) {
+      if (wcol0) mma_n(cur, std::integral_constant<int, (TBK + 1) / 2>{});
+      else mma_n(cur, std::integral_constant<int, TBK / 2>{});
+    } else {
+      mma_n(cur, std::integral_constant<int, FB>{});
+    }
+  };
+  const int fbw = NARROW ? (wcol0 ? (TBK + 1) / 2 : TBK / 2) : FB;   // blocks this wave stores
 
   // Phase prefetch (NPF > 0): every lane needs, for each of its outputs (m, n), the phase
   // of the complex element (n & ~1, n | 1) of row m. It loads the one float at its own
@@ -215,7 +246,7 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
 #pragma unroll
       for (int b = 0; b < FB; ++b)
         php[a][v][b] = *reinterpret_cast<const float*>(
-            Pc + rowoff + 4u * (unsigned)(n0 + wn + b * 16 + lr));
+            Pc + rowoff + 4u * (unsigned)band.map(n0 + wn + (b < fbw ? b : 0) * 16 + lr));
     }
   };
   // in_loop: issue the blocks due at loop step s; else every block the loop did not issue
@@ -271,7 +302,8 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
       const unsigned rowoff = 4u * (unsigned)m * (unsigned)Ncols;
 #pragma unroll
       for (int b = 0; b < FB; ++b) {
-        const unsigned n = (unsigned)(n0 + wn + b * 16 + lr);
+        if (NARROW && b >= fbw) continue;
+        const unsigned n = (unsigned)band.map(n0 + wn + b * 16 + lr);
         float val = acc[a][b][v] * rs;
         if (phase) {
           const float other = __shfl_xor(val, 1, 64);
@@ -328,6 +360,39 @@ __global__ __launch_bounds__(WM * WN * 64, (WM * WN > 4 ? 1 : TileOcc<BM, BN>::v
   const int lin = xcd_remap(blockIdx.x, tilesM * tilesN);
   gemm_tile<BM, BN, WM, WN, NPF>(sm, lin % tilesM, lin / tilesM, Yh, Yl, y_scale_exp, B, C, M, K,
                                  Ncols, alpha, rowscale, phase, bmax_in, cmax_out);
+}
+
+// Single-instance launch restricted to a column band: tiles of 256 rows x TBK 16-column
+// blocks (8 waves, one block per CU) over the virtual columns of `band`, `slots` tiles per
+// row panel. Afterwards every block writes zeros to its share of the real columns
+// [zero_lo, zero_hi) (the masked frequencies; empty range: no writes).
+template <int TBK, int NPF>
+__global__ __launch_bounds__(512, 1) void hankel_gemm_f16x3_band(
+    const h8* __restrict__ Yh, const h8* __restrict__ Yl, int y_scale_exp,
+    const float* __restrict__ B, float* __restrict__ C, int M, int K, int Ncols, float alpha,
+    const float* __restrict__ rowscale, const float* __restrict__ phase,
+    const unsigned* __restrict__ bmax_in, unsigned* __restrict__ cmax_out,
+    unsigned* __restrict__ reset_slot, int slots, ColBand band, int zero_lo, int zero_hi) {
+  __shared__ Smem<256, 128, 4, 2> sm;
+  if (reset_slot && blockIdx.x == 0 && threadIdx.x == 0) *reset_slot = 0u;
+  const int tilesM = M / 256;
+  const int nb = tilesM * slots;
+  const int lin = xcd_remap(blockIdx.x, nb);
+  const int tm = lin % tilesM, sl = lin / tilesM;
+  gemm_tile<256, 128, 4, 2, NPF, TBK>(sm, tm, 0, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
+                                      rowscale, phase, bmax_in, cmax_out, band,
+                                      sl * TBK * 16);
+  if (zero_hi > zero_lo) {
+    const int w4 = (zero_hi - zero_lo) / 4;
+    const long long total = (long long)M * w4;
+    const long long chunk = (total + gridDim.x - 1) / gridDim.x;
+    const long long end = min(total, (long long)(blockIdx.x + 1) * chunk);
+    for (long long i = (long long)blockIdx.x * chunk + threadIdx.x; i < end; i += 512) {
+      const int r = (int)(i / w4), c4 = (int)(i % w4);
+      *reinterpret_cast<float4*>(C + (size_t)r * Ncols + zero_lo + 4 * c4) =
+          make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
 }
 
 }  // namespace fpgemm16

@@ -839,6 +839,127 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
   return check_launch("hankel_gemm_mixed");
 }
 
+}  // extern "C"
+
+namespace {
+
+// Column band of the single-instance Hankel GEMMs: [zc_lo, zc_hi) are real columns whose
+// output is known to be zero (the masked frequencies). The band keeps the 16-column blocks
+// outside the largest 16-aligned part of that range; its blocks are split in tiles of TBK
+// blocks so that the tiles fill the CUs once (TBK = 4 .. 7; 8, i.e. the full tile width,
+// gains nothing and returns false). Padding blocks needed to fill the last tile are taken
+// from the masked range (they compute zeros).
+struct BandPlan {
+  int tbk, slots;
+  fpgemm16::ColBand band;
+  int zlo, zhi;   // real columns left to zero
+};
+
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
+
+// tile_rows x (TBK 16-column blocks) tiles, `per_cu` blocks resident per CU
+bool plan_band(int Nr, int Ncols, int zc_lo, int zc_hi, int cus, BandPlan* bp,
+               int tile_rows = 256, int per_cu = 1) {
+  if (Nr % tile_rows || zc_lo < 0 || zc_hi > Ncols || zc_hi <= zc_lo) return false;
+  const int a16 = (zc_lo + 15) / 16 * 16, b16 = zc_hi / 16 * 16;
+  if (b16 - a16 < 16) return false;
+  const int Lb = (a16 + (Ncols - b16)) / 16;
+  const int per_panel = cus * per_cu / (Nr / tile_rows);
+  if (per_panel < 1 || Lb < 1) return false;
+  const int tbk = (Lb + per_panel - 1) / per_panel;
+  if (tbk < 4 || tbk > 7) return false;
+  const int slots = (Lb + tbk - 1) / tbk;
+  const int va = a16 + (slots * tbk - Lb) * 16;
+  if (va > b16) return false;
+  bp->tbk = tbk;
+  bp->slots = slots;
+  bp->band = fpgemm16::ColBand{va, b16 - va};
+  bp->zlo = va;
+  bp->zhi = b16;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fp_hankel_gemm_band(const double* Yt, const double* B, double* C, int Nr, int Nt,
+                        double alpha, const double* rowscale, const double* phase, int zc_lo,
+                        int zc_hi, int zero_fill, void* stream) {
+  if (!Yt || !B || !C || Nr <= 0 || Nt <= 0) return fail(FP_E_ARG, "fp_hankel_gemm_band: bad argument");
+  if (B == C) return fail(FP_E_ARG, "fp_hankel_gemm_band: C must not alias B");
+  const int Ncols = 2 * Nt;
+  if (zc_lo < 0 || zc_hi > Ncols || (zc_hi > 0 && zc_hi < zc_lo))
+    return fail(FP_E_ARG, "fp_hankel_gemm_band: zero-column range outside [0, 2 Nt)");
+  BandPlan bp;
+  if (!plan_band(Nr, Ncols, zc_lo, zc_hi, device_cus(), &bp, fpgemm::BM, 2))
+    return fp_hankel_gemm(Yt, B, C, Nr, Nt, alpha, rowscale, phase, stream);
+  const int zlo = zero_fill ? bp.zlo : 0, zhi = zero_fill ? bp.zhi : 0;
+  const dim3 grid((Nr / fpgemm::BM) * bp.slots);
+  const fpgemm::Band64 band{bp.band.va, bp.band.shift};
+#define FP_BAND64(TB)                                                                          \
+  hipLaunchKernelGGL((fpgemm::hankel_gemm_f64_band<TB>), grid, dim3(fpgemm::NTHR), 0, S(stream), \
+                     Yt, B, C, Nr, Nr, Ncols, alpha, rowscale, phase, bp.slots, band, zlo, zhi); \
+  break;
+  switch (bp.tbk) {
+    case 4: FP_BAND64(4)
+    case 5: FP_BAND64(5)
+    case 6: FP_BAND64(6)
+    default: FP_BAND64(7)
+  }
+#undef FP_BAND64
+  return check_launch("hankel_gemm_band");
+}
+
+int fp_hankel_gemm_mixed_band(const void* Yh, const void* Yl, int y_scale_exp, const float* B,
+                              float* C, int Nr, int Nt, float alpha, const float* rowscale,
+                              const float* phase, const unsigned* bmax_in, unsigned* cmax_out,
+                              unsigned* reset_slot, int zc_lo, int zc_hi, int zero_fill,
+                              void* stream) {
+  if (!Yh || !Yl || !B || !C || !bmax_in)
+    return fail(FP_E_ARG, "fp_hankel_gemm_mixed_band: null pointer");
+  if (B == C) return fail(FP_E_ARG, "fp_hankel_gemm_mixed_band: C must not alias B");
+  const int Ncols = 2 * Nt;
+  if (Nr <= 0 || Nr % 128 || Ncols % 128)
+    return fail(FP_E_SHAPE, "fp_hankel_gemm_mixed_band: needs Nr % 128 == 0 and 2 Nt % 128 == 0");
+  if (zc_lo < 0 || zc_hi > Ncols || (zc_hi > 0 && zc_hi < zc_lo))
+    return fail(FP_E_ARG, "fp_hankel_gemm_mixed_band: zero-column range outside [0, 2 Nt)");
+  BandPlan bp;
+  if (!plan_band(Nr, Ncols, zc_lo, zc_hi, device_cus(), &bp))
+    return fp_hankel_gemm_mixed(Yh, Yl, y_scale_exp, B, C, Nr, Nt, alpha, rowscale, phase,
+                                bmax_in, cmax_out, reset_slot, stream);
+  const auto* yh = reinterpret_cast<const fpgemm16::h8*>(Yh);
+  const auto* yl = reinterpret_cast<const fpgemm16::h8*>(Yl);
+  const int zlo = zero_fill ? bp.zlo : 0, zhi = zero_fill ? bp.zhi : 0;
+  const dim3 grid((Nr / 256) * bp.slots);
+  constexpr int NPF = FP_MIXED_GEMM_NPF;
+#define FP_BAND(TB)                                                                            \
+  hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3_band<TB, NPF>), grid, dim3(512), 0, S(stream), \
+                     yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols, alpha, rowscale, phase, bmax_in,  \
+                     cmax_out, reset_slot, bp.slots, bp.band, zlo, zhi);                         \
+  break;
+  switch (bp.tbk) {
+    case 4: FP_BAND(4)
+    case 5: FP_BAND(5)
+    case 6: FP_BAND(6)
+    default: FP_BAND(7)
+  }
+#undef FP_BAND
+  return check_launch("hankel_gemm_mixed_band");
+}
+
 int fp_nl_split_c64(const float* u, const double* rho, const double* rho_s, float* rhs,
                     float* alpha, const fp_nl_params* p, const fp_nl_tables_f32* t,
                     void* stream) {

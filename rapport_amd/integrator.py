@@ -61,6 +61,8 @@ class Integrator:
         u0 = np.ascontiguousarray(np.asarray(u0).astype(np.complex128, copy=False))
         self.Nr, self.Nt = u0.shape
         u0d = torch.from_numpy(u0).to(dev)
+        from .mixed import zero_columns
+        self._zc = zero_columns(g["spec_mask"])   # real columns where the mask is exactly 0
         self._spec = None                       # FFT_t of the field (persistent buffer)
         self._u_last = None                     # (weakref, version) of the last `u` handed out
         self.rho = torch.zeros(self.Nr, self.Nt, dtype=torch.float64, device=dev)
@@ -233,15 +235,20 @@ class Integrator:
         call("fp_plasma_scan_spec", ptr(spec), ptr(self.rho), ptr(rho_s), None,
              ctypes.byref(pp), ptr(t.W_lut), ptr(t.Ws_lut), ptr(t.tw), ptr(t.status), st)
         phase = t.phase_half(dz)
-        call("fp_hankel_gemm", ptr(t.Yt), ptr(spec), ptr(w1), self.Nr, self.Nt, t.Rk, None,
-             ptr(phase), st)
-        call("fp_hankel_gemm", ptr(t.Yt), ptr(w1), ptr(w0), self.Nr, self.Nt, t.iRk, None, None, st)
+        # GEMMs only over the frequencies where the spectral mask is not 0 (the product is
+        # exactly 0 elsewhere); G2 / G4 write those zeros, as the row kernels read whole rows
+        zlo, zhi = self._zc
+        Nr, Nt = self.Nr, self.Nt
+        call("fp_hankel_gemm_band", ptr(t.Yt), ptr(spec), ptr(w1), Nr, Nt, t.Rk, None,
+             ptr(phase), zlo, zhi, 0, st)
+        call("fp_hankel_gemm_band", ptr(t.Yt), ptr(w1), ptr(w0), Nr, Nt, t.iRk, None, None,
+             zlo, zhi, 1, st)
         call("fp_nl_step", ptr(w0), ptr(w0), ptr(self.rho), ptr(self._rho_s_arg()), 1, 1,
              ctypes.byref(self.nl.params(dz)), ctypes.byref(t.tables), st)
-        call("fp_hankel_gemm", ptr(t.Yt), ptr(w0), ptr(w1), self.Nr, self.Nt, t.Rk, None,
-             ptr(phase), st)
-        call("fp_hankel_gemm", ptr(t.Yt), ptr(w1), ptr(spec), self.Nr, self.Nt, t.iRk,
-             ptr(t.mask_r), None, st)
+        call("fp_hankel_gemm_band", ptr(t.Yt), ptr(w0), ptr(w1), Nr, Nt, t.Rk, None,
+             ptr(phase), zlo, zhi, 0, st)
+        call("fp_hankel_gemm_band", ptr(t.Yt), ptr(w1), ptr(spec), Nr, Nt, t.iRk,
+             ptr(t.mask_r), None, zlo, zhi, 1, st)
 
     def capture_step(self, steps=1):
         """Capture `steps` z-steps (plasma scan + Strang step, as propagate() runs them)

@@ -113,6 +113,18 @@ def matched_scale_pair(x, search=4096):
     return best[1], best[2]
 
 
+def zero_columns(spec_mask):
+    """Real-column range [lo, hi) of the frequencies where the spectral mask is exactly 0
+    (the longest contiguous run; (0, 0) if none): every Hankel GEMM output of the step is
+    exactly zero there (the half-step phase carries the mask)."""
+    z = np.flatnonzero(np.asarray(spec_mask) == 0.0)
+    if z.size == 0:
+        return 0, 0
+    runs = np.split(z, np.flatnonzero(np.diff(z) != 1) + 1)
+    r = max(runs, key=len)
+    return 2 * int(r[0]), 2 * (int(r[-1]) + 1)
+
+
 class MixedTables:
     """Float32 / FP16 device tables of the mixed path, derived from the FP64 ones."""
 
@@ -145,6 +157,7 @@ class MixedTables:
         mr[:self.Nr] = g["mask_r_1d"]
         self.mask_r = f32(mr)
         self.Rk, self.iRk = matched_scale_pair(t64.Rk)
+        self.zc = zero_columns(g["spec_mask"])
         self._phase = {}
         self.tables = _lib.NLTablesF32(ptr(self.tw), ptr(self.mA), ptr(self.mB), ptr(self.mC),
                                        ptr(self.Rf), ptr(self.knots), ptr(self.inv_h),
@@ -219,15 +232,18 @@ class MixedEngine:
             self.slots[1:].zero_()
         Nr, Nt = self.Np, self.Nt          # the GEMMs run over the padded rows
         ph = m.phase_half(dz)
-        g = lambda B, C, a, rs, phase, bin_, cout, reset: call(  # noqa: E731
-            "fp_hankel_gemm_mixed", ptr(m.Yh), ptr(m.Yl), m.y_scale_exp, ptr(B), ptr(C), Nr, Nt,
-            ctypes.c_float(a), ptr(rs), ptr(phase), bin_, cout, reset, st)
-        g(self.spec, self.w1, m.Rk, None, ph, self._slot(0), self._slot(1), None)
-        g(self.w1, self.w0, m.iRk, None, None, self._slot(1), None, self._slot(0))
+        zlo, zhi = m.zc
+        # G1 / G3 leave the masked columns of w1 unwritten (G2 / G4 do not read them); G2 /
+        # G4 write zeros there, since the nonlinear step and the carrier scan read whole rows
+        g = lambda B, C, a, rs, phase, bin_, cout, reset, zf: call(  # noqa: E731
+            "fp_hankel_gemm_mixed_band", ptr(m.Yh), ptr(m.Yl), m.y_scale_exp, ptr(B), ptr(C), Nr,
+            Nt, ctypes.c_float(a), ptr(rs), ptr(phase), bin_, cout, reset, zlo, zhi, zf, st)
+        g(self.spec, self.w1, m.Rk, None, ph, self._slot(0), self._slot(1), None, 0)
+        g(self.w1, self.w0, m.iRk, None, None, self._slot(1), None, self._slot(0), 1)
         call("fp_nl_step_c64", ptr(self.w0), ptr(self.w0), ptr(rho), ptr(rho_s_arg), 1, 1,
              ctypes.byref(nl_params), ctypes.byref(m.tables), self._slot(2), st)
-        g(self.w0, self.w1, m.Rk, None, ph, self._slot(2), self._slot(3), None)
-        g(self.w1, self.spec, m.iRk, m.mask_r, None, self._slot(3), self._slot(0), None)
+        g(self.w0, self.w1, m.Rk, None, ph, self._slot(2), self._slot(3), None, 0)
+        g(self.w1, self.spec, m.iRk, m.mask_r, None, self._slot(3), self._slot(0), None, 1)
 
     def step(self, dz, rho, rho_s, rho_s_arg, nl_params):
         self.plasma(rho, rho_s)

@@ -118,6 +118,49 @@ def test_hankel_gemm(Nr, Nt):
     assert np.abs(got - ref2).max() / np.abs(ref2).max() < 1e-12
 
 
+@pytest.mark.parametrize("Nr,Nt,zc,expect_band", [(1024, 4096, (4096, 6048), True),
+                                                  (512, 2048, (1000, 2900), False),
+                                                  (768, 4096, (3500, 5000), True),
+                                                  (200, 512, (300, 700), False)])
+def test_hankel_gemm_band_equals_full(Nr, Nt, zc, expect_band):
+    # the banded FP64 GEMM (tiles only over the columns outside the masked range [zc)) gives
+    # bitwise fp_hankel_gemm's values there, with and without phase / row scale; zero_fill
+    # writes 0 in the masked columns, otherwise they keep their contents (16-column
+    # alignment: the range's ends are computed). Nr = 200 is not tiled by 128 rows: the entry
+    # falls back to the full GEMM.
+    from rapport_amd import _lib
+    _announce(f"hankel_gemm_band Nr={Nr} Nt={Nt} zero columns {zc}")
+    rng = np.random.default_rng(Nr + 1)
+    Y = rng.standard_normal((Nr, Nr))
+    B = rng.standard_normal((Nr, Nt)) + 1j * rng.standard_normal((Nr, Nt))
+    ph = np.exp(1j * rng.uniform(0, 2 * np.pi, (Nr, Nt)))
+    rs = rng.uniform(0.5, 1.5, Nr)
+    Yt, Bd = _cuda(Y.T), _cuda(B)
+    phd, rsd = _cuda(ph), _cuda(rs)
+    live = np.ones(2 * Nt, bool)
+    live[zc[0]:zc[1]] = False
+    for rsp, php in ((None, None), (rsd, phd)):
+        full = torch.empty(Nr, Nt, dtype=torch.complex128, device="cuda")
+        _lib.call("fp_hankel_gemm", _lib.ptr(Yt), _lib.ptr(Bd), _lib.ptr(full), Nr, Nt, 0.75,
+                  _lib.ptr(rsp), _lib.ptr(php), _lib.stream_ptr())
+        for zf in (0, 1):
+            band = torch.full((Nr, Nt), complex(7.0, 7.0), dtype=torch.complex128, device="cuda")
+            _lib.call("fp_hankel_gemm_band", _lib.ptr(Yt), _lib.ptr(Bd), _lib.ptr(band), Nr, Nt,
+                      0.75, _lib.ptr(rsp), _lib.ptr(php), zc[0], zc[1], zf, _lib.stream_ptr())
+            f = full.cpu().numpy().view(np.float64)
+            b = band.cpu().numpy().view(np.float64)
+            assert np.array_equal(b[:, live], f[:, live]), (zf, rsp is None)
+            dead = b[:, ~live]
+            fd = f[:, ~live]
+            # a column of the range is computed (alignment, padding of the last tile -- the
+            # pipeline's data are zero there -- or the full-GEMM fallback when the band would
+            # not fill the GPU better), zeroed (zero_fill) or left untouched
+            mark = 0.0 if zf else 7.0
+            assert np.all((dead == mark) | (dead == fd))
+            if expect_band:
+                assert (dead == mark).any()
+
+
 @pytest.mark.parametrize("case,ste,peaks", [
     (C1, False, (1e12, 3e13, 6e13, 9e13)),
     (C1, True, (1e12, 3e13, 6e13, 9e13)),

@@ -165,3 +165,53 @@ def test_reference_production_grid_n3001_mixed_vs_fp64():
     _announce(f"  fluence {e_f:.2e} field {e_u:.2e} rho {e_r:.2e}, max rho {rp['rho_rz'].max():.2e}")
     assert rp["rho_rz"].max() > 1e15 and bool(torch.isfinite(ip.u).all())
     assert e_f < 1e-4 and e_r < 1e-3 and e_u < 1e-3
+
+
+@pytest.mark.parametrize("case", [C2, dict(C2, N=769), dict(C2, N=513)])
+def test_band_gemm_equals_full_gemm_outside_the_mask(case):
+    # the banded single-instance GEMM (tiles only over the frequencies where the spectral mask
+    # is not exactly 0: 7-, 5- and 4-block tiles at Nr = 1024, 768, 512) gives bitwise the
+    # full GEMM's values there; with zero_fill the masked columns are written as 0, without
+    # it they are left as they were.
+    import rapport_amd as R
+    from rapport_amd import _lib
+    from rapport_amd.mixed import split_hankel_fp16, zero_columns
+    _announce(f"band GEMM vs full GEMM, N={case['N']} Nt={case['Nt']}")
+    cfg = R.Config(**cfg_kwargs(case))
+    g = R.build_grids(cfg)
+    Y = g["Y"]
+    Nr, Nt = Y.shape[0], cfg.Nt
+    zlo, zhi = zero_columns(g["spec_mask"])
+    assert zhi - zlo > 1000
+    rng = np.random.default_rng(5)
+    B = (rng.standard_normal((Nr, Nt)) + 1j * rng.standard_normal((Nr, Nt))) * 3e9
+    Bd = _cuda(B.astype(np.complex64))
+    yh, yl, se = split_hankel_fp16(Y)
+    Yh, Yl = _cuda(yh), _cuda(yl)
+    ph = _cuda((np.exp(1j * rng.uniform(0, 2 * np.pi, (Nr, Nt))) * g["spec_mask"][None, :])
+               .astype(np.complex64))
+    slots = torch.zeros(4, dtype=torch.int32, device="cuda")
+    sp = lambda i: ctypes.c_void_p(slots.data_ptr() + 4 * i)  # noqa: E731
+    st = _lib.stream_ptr()
+    _lib.call("fp_absmax_f32", _lib.ptr(Bd), 2 * Nr * Nt, sp(0), st)
+    full = torch.full_like(Bd, complex(7.0, 7.0))
+    _lib.call("fp_hankel_gemm_mixed", _lib.ptr(Yh), _lib.ptr(Yl), se, _lib.ptr(Bd), _lib.ptr(full),
+              Nr, Nt, ctypes.c_float(0.5), None, _lib.ptr(ph), sp(0), sp(1), None, st)
+    for zf in (0, 1):
+        band = torch.full_like(Bd, complex(7.0, 7.0))
+        _lib.call("fp_hankel_gemm_mixed_band", _lib.ptr(Yh), _lib.ptr(Yl), se, _lib.ptr(Bd),
+                  _lib.ptr(band), Nr, Nt, ctypes.c_float(0.5), None, _lib.ptr(ph), sp(0), sp(2),
+                  None, zlo, zhi, zf, st)
+        torch.cuda.synchronize()
+        f = torch.view_as_real(full).reshape(Nr, 2 * Nt)
+        b = torch.view_as_real(band).reshape(Nr, 2 * Nt)
+        live = torch.ones(2 * Nt, dtype=torch.bool, device="cuda")
+        live[zlo:zhi] = False
+        assert torch.equal(b[:, live], f[:, live]), zf
+        dead = b[:, ~live]
+        if zf:
+            assert bool((dead == 0).all())
+        else:
+            # 16-column alignment: the ends of the masked range may be computed (zeros)
+            assert bool(((dead == 7.0) | (dead == 0)).all()) and bool((dead == 7.0).any())
+        assert slots[2].item() == slots[1].item()   # same max|C|
