@@ -330,7 +330,22 @@ def run_batched(n_inst=64, steps=20, seed=0, tune=True):
                tuned={k: v for k, v in b.config.items() if k != "timings_ms"},
                finite=finite, capped_rows=sum(int(m.t.status.item()) for m in b.members))
     b.close()
-    del b, g
+    # the same sweep with one launch per pass (six per z-step)
+    bm = BatchedIntegrator(cfgs, envs)
+    bm.set_merged(True)
+    gm = bm.capture(1)
+    for _ in range(2):
+        gm.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gm.replay()
+    torch.cuda.synchronize()
+    dtm = (time.perf_counter() - t0) / steps
+    out["single_launch_per_pass"] = dict(ms_per_step=1e3 * dtm, inst_steps_per_s=n_inst / dtm,
+                                         launches_per_step=bm.launches_per_step())
+    bm.close()
+    del b, g, bm, gm
     torch.cuda.empty_cache()
     return out
 

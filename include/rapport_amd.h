@@ -382,6 +382,14 @@ int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb);
  * block per row (tile); > 0 = that many persistent blocks, each looping over work items. */
 int fp_batch_set_grid(fp_batch_plan* plan, int kind, int blocks);
 
+/* merged = 1: every pass is one launch for the whole batch (all Nt classes, STE flags and
+ * tile shapes in one work list; 256 x 128 GEMM tiles where Nr % 256 == 0, else 128 x 256;
+ * nonlinear step with Nt / 8 threads per row; persistent grids sized to the device), so a
+ * z-step is six launches; the per-instance tiles, threads and grids of fp_batch_set_config /
+ * fp_batch_set_grid then do not apply. Needs Nr % 128 == 0. merged = 0: one launch per
+ * group of instances sharing a kernel configuration (the default). */
+int fp_batch_set_merged(fp_batch_plan* plan, int merged);
+
 /* One pass (FP_PASS_*) for every instance, or the whole z-step (fp_batch_step_mixed). */
 int fp_batch_pass(fp_batch_plan* plan, int pass, void* stream);
 int fp_batch_step_mixed(fp_batch_plan* plan, void* stream);

@@ -110,6 +110,13 @@ class BatchedIntegrator:
         call("fp_batch_set_grid", self._plan, KIND[kind], int(blocks))
         self._graph = None
 
+    def set_merged(self, merged=True):
+        """One launch per pass for the whole batch (six launches per z-step) instead of
+        one per group of instances sharing a kernel configuration."""
+        call("fp_batch_set_merged", self._plan, int(bool(merged)))
+        self.merged = bool(merged)
+        self._graph = None
+
     def launches_per_step(self):
         return int(_lib.load().fp_batch_launches(self._plan))
 

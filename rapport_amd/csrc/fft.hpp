@@ -235,7 +235,7 @@ FP_DEV void row_fft(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict_
   // Opaque copy of the thread index: keeps the compiler from hoisting this
   // transform's LDS / twiddle addresses out of enclosing loops and holding
   // them in registers across the whole kernel.
-  int tid = (int)threadIdx.x;
+  int tid = (int)threadIdx.x & (TPB - 1);
   asm volatile("" : "+v"(tid));
   passes<T, N, TPB, INV, 1, EXACT>(v, lds, tw, tid);
 }

@@ -84,7 +84,7 @@ FP_DEV void split_row(const c32 (&us)[NT / TPB], const float* rho_row,
                       const double* __restrict__ rhos_row, c32 (&k)[NT / TPB], float* al,
                       const fp_nl_params& p, const fp_nl_tables_f32& t, c32* lds) {
   constexpr int E = NT / TPB;
-  int tid = threadIdx.x;
+  int tid = threadIdx.x & (TPB - 1);
   asm volatile("" : "+v"(tid));
   const c32* tw = reinterpret_cast<const c32*>(t.tw);
   const c32* Rf = reinterpret_cast<const c32*>(t.Rf);
@@ -160,13 +160,13 @@ template <int NT, int TPB>
 FP_DEV void load_row(const float* __restrict__ src, size_t row, c32 (&v)[NT / TPB]) {
   const c32* r = reinterpret_cast<const c32*>(src) + row * NT;
 #pragma unroll
-  for (int s = 0; s < NT / TPB; ++s) v[s] = r[threadIdx.x + s * TPB];
+  for (int s = 0; s < NT / TPB; ++s) v[s] = r[(threadIdx.x & (TPB - 1)) + s * TPB];
 }
 template <int NT, int TPB>
 FP_DEV void store_row(float* __restrict__ dst, size_t row, const c32 (&v)[NT / TPB]) {
   c32* r = reinterpret_cast<c32*>(dst) + row * NT;
 #pragma unroll
-  for (int s = 0; s < NT / TPB; ++s) r[threadIdx.x + s * TPB] = v[s];
+  for (int s = 0; s < NT / TPB; ++s) r[(threadIdx.x & (TPB - 1)) + s * TPB] = v[s];
 }
 
 template <int NT, int TPB, bool RS>
@@ -184,13 +184,13 @@ __global__ __launch_bounds__(TPB) void split_kernel(const float* __restrict__ u,
   float al[E];
   load_row<NT, TPB>(u, row, us);
 #pragma unroll
-  for (int s = 0; s < E; ++s) sr[threadIdx.x + s * TPB] = (float)rho[row * NT + threadIdx.x + s * TPB];
+  for (int s = 0; s < E; ++s) sr[(threadIdx.x & (TPB - 1)) + s * TPB] = (float)rho[row * NT + (threadIdx.x & (TPB - 1)) + s * TPB];
   __syncthreads();
   split_row<NT, TPB, RS>(us, sr, RS ? rhos + row * NT : nullptr, k, al, p, t, lds);
   store_row<NT, TPB>(rhs, row, k);
   if (alpha) {
 #pragma unroll
-    for (int s = 0; s < E; ++s) alpha[row * NT + threadIdx.x + s * TPB] = al[s];
+    for (int s = 0; s < E; ++s) alpha[row * NT + (threadIdx.x & (TPB - 1)) + s * TPB] = al[s];
   }
 }
 
@@ -202,7 +202,7 @@ FP_DEV void alpha_half(c32 (&u)[E], const float* rho_row, const fp_nl_params& p,
   for (int s = 0; s < E; ++s) {
     const float a2 = u[s].x * u[s].x + u[s].y * u[s].y;
     float ph;
-    const float a = alpha_of(a2, rho_row[threadIdx.x + s * TPB], p, t, ph);
+    const float a = alpha_of(a2, rho_row[(threadIdx.x & (TPB - 1)) + s * TPB], p, t, ph);
     u[s] = expf(hdz * a) * u[s];
   }
 }
@@ -222,7 +222,7 @@ FP_DEV void nl_step_row(StepSmem<NT>& sm, size_t row, const float* in, float* ou
                         int in_spectral, int out_spectral, const fp_nl_params& p,
                         const fp_nl_tables_f32& t, unsigned* __restrict__ cmax_out) {
   constexpr int E = NT / TPB;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x & (TPB - 1);
   c32* lds = sm.lds;
   c32* ub = sm.ub;
   float* sr = sm.sr;

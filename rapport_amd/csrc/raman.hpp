@@ -45,7 +45,7 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   constexpr int E = NT / TPB;
   static_assert(E % 2 == 0, "the N/2 shift must stay inside a thread");
   static_assert(TPB % 64 == 0, "whole waves");
-  int tid = threadIdx.x;
+  int tid = threadIdx.x & (TPB - 1);
   asm volatile("" : "+v"(tid));
   const int lane = tid & 63, wave = tid >> 6;
   const T C = wt[NT + 1].x;

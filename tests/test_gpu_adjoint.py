@@ -183,3 +183,45 @@ def test_fit_lowers_loss(pair):
     _, hist = AD.fit_lg(eng, F_t, torch.tensor(theta), n, iters=15)
     _announce(f"  loss {hist[0]:.3e} -> {min(hist):.3e} in {len(hist)} evaluations")
     assert min(hist) <= 0.1 * hist[0]
+
+
+# the adjoint-fit workload (512 x 2048, 1000 steps of 10 um, 3 uJ, 85 fs, w0 = 100 um)
+C5 = dict(nz=999, Nt=2048, N=513, R_factor=15.0, begin=0.0, end=999 * 10e-6, energy_uJ=3.0,
+          wavelength=800e-9, n2=3.2e-20, Ui_eV=9.0, rho_max=2.1e22, tau_c=1.27e-15,
+          tau_r=150e-15, delta_t=85e-15, w0=100e-6, enable_ste=False, save_stride=10 ** 6,
+          rho_t_stride=0, write_output=False, verbose=False)
+
+
+def test_c5_gradient_vs_fp64_oracle_fixture():
+    """Gradient of the fluence mismatch w.r.t. 32 Laguerre-Gauss coefficients after 1000
+    steps at 512 x 2048 (mixed forward with sqrt(n) checkpoints + hand-written adjoint)
+    against the FP64 oracle's discrete adjoint on the same problem (tests/golden/
+    c5_gradient.npz, written by tests/golden/make_c5_gradient.py). Bars: loss 1e-3 and
+    gradient rel-L2 2e-3 (measured 1.2e-4 and 3.0e-4: the FP32 forward differs from the FP64
+    one by its rounding, accumulated over 1000 steps; the gradient is that of the FP32
+    forward)."""
+    import os
+    import rapport_amd as R
+    from rapport_amd import adjoint as AD
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "c5_gradient.npz"))
+    P, n = int(gold["P"]), int(gold["steps"])
+    _announce(f"C5 gradient: {2 * P} LG coefficients, {n} steps at 512 x 2048 vs FP64 fixture")
+    it = R.Integrator(R.Config(**C5, precision="mixed"))
+    eng = AD.AdjointEngine(it)
+    dev = it.device
+    G = AD.lg_modes(it.g["rlist"], it.cfg.w0, P).to(dev)
+    tenv = AD.time_envelope(it.g, it.cfg, dev)
+    w = AD.radial_weights(it.g, dev)
+    th = torch.tensor(gold["theta"], dtype=torch.float64, device=dev, requires_grad=True)
+    F = AD.fluence_from_input(AD.lg_field(th, G, float(it.cfg.E0), tenv), eng, n)
+    loss = AD.fluence_loss(F, torch.from_numpy(gold["F_target"]).to(dev), w)
+    loss.backward()
+    L = float(loss.detach())
+    g = th.grad.cpu().numpy()
+    eL = abs(L - float(gold["loss"])) / float(gold["loss"])
+    e = rel_l2(g, gold["grad"])
+    cos = float(np.dot(g, gold["grad"]) / np.linalg.norm(g) / np.linalg.norm(gold["grad"]))
+    _announce(f"  loss {L:.6e} vs {float(gold['loss']):.6e} (rel {eL:.1e}); gradient rel-L2 "
+              f"{e:.2e}, cosine {cos:.6f}")
+    assert eL <= 1e-3
+    assert e <= 2e-3

@@ -138,3 +138,17 @@ def test_sweep_smallest_classes_vs_fp64_oracle(case):
         _announce(f"  field {eu:.2e} fluence {ef:.2e} rho {er:.2e}")
         assert eu <= 1e-4 and ef <= 1e-4 and er <= 1e-4
         b.close()
+
+
+def test_sweep_classes_merged_single_launch_per_pass_vs_single(case):
+    from rapport_amd.batch import BatchedIntegrator
+    cfgs, envs, singles = case
+    _announce("sweep classes: one launch per pass (merged work lists), graph, 3 steps")
+    b = BatchedIntegrator(cfgs, envs)
+    b.set_merged(True)
+    assert b.launches_per_step() == 6
+    g = b.capture(1)
+    for _ in range(STEPS):
+        g.replay()
+    _check(b, singles, "merged")
+    b.close()
