@@ -297,31 +297,38 @@ int fp_nl_step_c64(const float* in, float* out, const double* rho, const double*
 
 /* ---------------------------------------------------------------- adjoint
  * No counterpart in the reference (it has no gradients): the discrete adjoint of
- * one mixed-precision z-step, STE off. Gradients use the convention
+ * one mixed-precision z-step, with or without STE. Gradients use the convention
  * dL = Re sum conj(g) du and travel between calls as FFT_t(g) ("spectral"), like
  * the field state. With the split-FP16 GEMM applied to Y^T (and Y^T diag(mask_r))
  * and the conjugate half-step phase, one adjoint step is
  *   A1 = iRk (Y^T diag(mask_r) gamma) x conj(phase), g4 = Rk Y^T A1,
- *   fp_adj_nl_step_c64 (g4 -> g1, dL/drho),
+ *   fp_adj_nl_step_c64 (g4 -> g1, dL/drho, dL/drho_s),
  *   A3 = iRk (Y^T g1) x conj(phase), gamma_n = Rk Y^T A3,
  *   fp_adj_plasma_c64 (gamma_n += FFT_t(2 invE2 dL/dI u_n)). */
 
 /* Adjoint of the nonlinear half of Integrator.step (integrator.py:142-146) at the
  * state v1s (spectral, the field after the first half linear step; rho = the
- * carrier densities of the step). g4s: spectral gradient w.r.t. the input of the
- * second half linear step; g1s (may alias g4s): spectral gradient w.r.t. v1s;
- * grho: float Nr x Nt dL/drho; scratch: 4 Nr x Nt complex64 planes; cmax_out
- * (optional): max |re|, |im| of g1s (float bits, atomicMax). */
-int fp_adj_nl_step_c64(const float* v1s, const double* rho, const float* g4s, float* g1s,
-                       float* grho, float* scratch, const fp_nl_params* p,
-                       const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream);
+ * carrier densities of the step). rho_s: the step's STE density when the forward
+ * step applied the STE index term (the rho_s argument of fp_nl_step_c64), else
+ * NULL; grho_s (required with rho_s): float Nr x Nt dL/drho_s. g4s: spectral
+ * gradient w.r.t. the input of the second half linear step; g1s (may alias g4s):
+ * spectral gradient w.r.t. v1s; grho: float Nr x Nt dL/drho; scratch: 4 Nr x Nt
+ * complex64 planes; cmax_out (optional): max |re|, |im| of g1s (float bits,
+ * atomicMax). */
+int fp_adj_nl_step_c64(const float* v1s, const double* rho, const double* rho_s,
+                       const float* g4s, float* g1s, float* grho, float* grho_s, float* scratch,
+                       const fp_nl_params* p, const fp_nl_tables_f32* t, unsigned* cmax_out,
+                       void* stream);
 
-/* Adjoint of the carrier scan (fp_plasma_scan_spec_c64, kernels.py:36-106, STE off):
- * us = the spectral state the scan read, rho = its result, grho = dL/drho; adds
+/* Adjoint of the carrier scan (fp_plasma_scan_spec_c64, kernels.py:36-106):
+ * us = the spectral state the scan read, rho (and, with p->enable_ste, rho_s) = its
+ * result, grho (grho_s) = dL/drho (dL/drho_s); W_lut, Ws_lut: the scan's rate
+ * tables (rho_s, grho_s, Ws_lut may be NULL with STE off); adds
  * FFT_t(2 invE2 (dL/dI) u) to the spectral gradient gs; cmax_out (optional): max of
  * the updated gs. */
-int fp_adj_plasma_c64(const float* us, const double* rho, const float* grho, float* gs,
-                      const fp_plasma_params* p, const double* W_lut, const float* tw,
+int fp_adj_plasma_c64(const float* us, const double* rho, const double* rho_s, const float* grho,
+                      const float* grho_s, float* gs, const fp_plasma_params* p,
+                      const double* W_lut, const double* Ws_lut, const float* tw,
                       unsigned* cmax_out, void* stream);
 
 /* ---------------------------------------------------------------- batched form

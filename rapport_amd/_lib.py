@@ -99,9 +99,9 @@ _SIGS = {
     "fp_nl_split_c64": [P, P, P, P, P, ctypes.POINTER(NLParams), ctypes.POINTER(NLTablesF32), P],
     "fp_nl_step_c64": [P, P, P, P, c_int, c_int, ctypes.POINTER(NLParams),
                        ctypes.POINTER(NLTablesF32), P, P],
-    "fp_adj_nl_step_c64": [P, P, P, P, P, P, ctypes.POINTER(NLParams),
+    "fp_adj_nl_step_c64": [P, P, P, P, P, P, P, P, ctypes.POINTER(NLParams),
                            ctypes.POINTER(NLTablesF32), P, P],
-    "fp_adj_plasma_c64": [P, P, P, P, ctypes.POINTER(PlasmaParams), P, P, P, P],
+    "fp_adj_plasma_c64": [P, P, P, P, P, P, ctypes.POINTER(PlasmaParams), P, P, P, P, P],
     "fp_batch_plan_create": [ctypes.POINTER(BatchInstance), c_int, ctypes.POINTER(c_void_p)],
     "fp_batch_plan_destroy": [P],
     "fp_batch_set_config": [P, c_int, c_int, c_int, c_int],

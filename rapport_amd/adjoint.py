@@ -1,22 +1,27 @@
 """Differentiation form: hand-written adjoint of the mixed-precision z-step.
 
-``AdjointEngine`` wraps a mixed-precision ``Integrator`` (STE off) and computes the
+``AdjointEngine`` wraps a mixed-precision ``Integrator`` and computes the
 output-plane fluence F(r) = sum_t |u(r, t)|^2 dt invE2 after n z-steps (the
 propagate() loop body: carrier scan + Strang step) and the gradient of a loss of F
 with respect to the input field, by the discrete adjoint of every step:
 
     gamma <- A1 = iRk (Y^T diag(mask_r) gamma) x conj(phase)   [split-FP16 GEMM]
              g4 = Rk Y^T A1                                     [split-FP16 GEMM]
-             fp_adj_nl_step_c64: g4 -> g1 (+ dL/drho)           [row kernel]
+             fp_adj_nl_step_c64: g4 -> g1 (+ dL/drho, dL/drho_s) [row kernel]
              A3 = iRk (Y^T g1) x conj(phase); gamma = Rk Y^T A3 [GEMMs]
              fp_adj_plasma_c64: gamma += FFT_t(2 invE2 dL/dI u)  [row kernel]
+
+With STE on (``enable_ste``), the carrier scan's adjoint runs over the coupled
+(n_e, n_s) state and the nonlinear step's adjoint differentiates the STE index term
+i ste_pref rho_s u as well (dL/drho_s feeds the scan's adjoint).
 
 Gradients travel as FFT_t of the real-gradient field g (dL = Re sum conj(g) du),
 the same representation as the spectral field state. The forward states are
 checkpointed every ``ckpt`` steps (default ceil(sqrt(n))) and each segment is
 recomputed once during the backward sweep, keeping per step the state, its carrier
 densities and the field after the first half linear step (what the step's adjoint
-linearises around), so memory holds about sqrt(n) states + 3 sqrt(n) step records.
+linearises around), so memory holds about sqrt(n) states + 3 sqrt(n) step records
+(4 with STE: rho_s too).
 
 ``FluenceFromInput`` exposes the forward / backward pair as a torch.autograd.Function
 (input: complex64 time-domain field, output: F as float64). ``lg_modes`` /
@@ -39,8 +44,6 @@ class AdjointEngine:
     def __init__(self, integ, ckpt=None):
         if integ._mx is None:
             raise ValueError("AdjointEngine needs a mixed-precision Integrator (precision='mixed')")
-        if integ.cfg.enable_ste:
-            raise ValueError("the adjoint covers STE off")
         if integ._mx.Np != integ.Nr:
             raise ValueError(f"the adjoint needs Nr % 128 == 0 (Nr = {integ.Nr})")
         self.it = integ
@@ -64,6 +67,10 @@ class AdjointEngine:
         self.a1 = torch.zeros(shp, **c64)
         self.g4 = torch.zeros(shp, **c64)
         self.grho = torch.zeros(shp, dtype=torch.float32, device=dev)
+        self.ste = bool(integ.cfg.enable_ste)
+        # dL/drho_s: written by the nonlinear adjoint when the forward applies the STE
+        # index term, otherwise it stays zero (the scan's adjoint still reads it)
+        self.grho_s = torch.zeros(shp, dtype=torch.float32, device=dev) if self.ste else None
         self.scratch = torch.zeros((4,) + shp, **c64)
         self.aslots = torch.zeros(4, dtype=torch.int32, device=dev)
         self.dt_invE2 = float(g["dt"] * g["invE2"])
@@ -84,7 +91,7 @@ class AdjointEngine:
 
     def _step(self):
         it = self.it
-        self.mx.step(self.dz, it.rho, it.rho_s, None, self.nlp)
+        self.mx.step(self.dz, it.rho, it.rho_s, it._rho_s_arg(), self.nlp)
 
     def fluence(self):
         """F(r) of the current state (float64, Parseval over the spectral rows)."""
@@ -105,19 +112,20 @@ class AdjointEngine:
 
     def _forward_keep(self, finish):
         """One forward step from mx.spec, keeping what its adjoint needs: the state
-        (spectral), its carrier densities and v1 = the field after the first half
-        linear step (spectral). With finish=False the step stops after v1."""
+        (spectral), its carrier densities (rho, and rho_s with STE) and v1 = the field
+        after the first half linear step (spectral). With finish=False the step stops
+        after v1."""
         mx, it, st = self.mx, self.it, stream_ptr()
         mt = mx.m
         u = mx.spec.clone()
-        mx.plasma(it.rho, None)                                          # clears slots 1-3
+        mx.plasma(it.rho, it.rho_s)                                      # clears slots 1-3
         self._gemm(mt.Yh, mt.Yl, mt.y_scale_exp, mx.spec, mx.w1, mt.Rk, self.phase,
                    mx._slot(0), mx._slot(1), None)
         self._gemm(mt.Yh, mt.Yl, mt.y_scale_exp, mx.w1, mx.w0, mt.iRk, None,
                    mx._slot(1), None, mx._slot(0))
-        keep = (u, it.rho.clone(), mx.w0.clone())
+        keep = (u, it.rho.clone(), mx.w0.clone(), it.rho_s.clone() if self.ste else None)
         if finish:
-            call("fp_nl_step_c64", ptr(mx.w0), ptr(mx.w0), ptr(it.rho), None, 1, 1,
+            call("fp_nl_step_c64", ptr(mx.w0), ptr(mx.w0), ptr(it.rho), ptr(it._rho_s_arg()), 1, 1,
                  ctypes.byref(self.nlp), ctypes.byref(mt.tables), mx._slot(2), st)
             self._gemm(mt.Yh, mt.Yl, mt.y_scale_exp, mx.w0, mx.w1, mt.Rk, self.phase,
                        mx._slot(2), mx._slot(3), None)
@@ -126,10 +134,10 @@ class AdjointEngine:
                  mx._slot(3), mx._slot(0), None, st)
         return keep
 
-    def _adjoint_step(self, un, rho_n, v1_n):
+    def _adjoint_step(self, un, rho_n, v1_n, rhos_n):
         """gam <- gradient w.r.t. the state un (spectral) given gam w.r.t. step(un);
-        rho_n, v1_n: that step's carrier densities and field after the first half
-        linear step (spectral)."""
+        rho_n, v1_n, rhos_n: that step's carrier densities, field after the first half
+        linear step (spectral) and STE density (None with STE off)."""
         it, st = self.it, stream_ptr()
         t = it.t
         mt = self.mx.m
@@ -138,15 +146,18 @@ class AdjointEngine:
                    self._aslot(0), self._aslot(1), None)
         self._gemm(self.YTh, self.YTl, self.yt_se, self.a1, self.g4, mt.Rk, None,
                    self._aslot(1), None, self._aslot(0))
-        call("fp_adj_nl_step_c64", ptr(v1_n), ptr(rho_n), ptr(self.g4), ptr(self.g4),
-             ptr(self.grho), ptr(self.scratch), ctypes.byref(self.nlp), ctypes.byref(mt.tables),
+        rs_arg = rhos_n if it._rho_s_arg() is not None else None
+        call("fp_adj_nl_step_c64", ptr(v1_n), ptr(rho_n), ptr(rs_arg), ptr(self.g4),
+             ptr(self.g4), ptr(self.grho), ptr(self.grho_s if rs_arg is not None else None),
+             ptr(self.scratch), ctypes.byref(self.nlp), ctypes.byref(mt.tables),
              self._aslot(2), st)
         self._gemm(self.YTh, self.YTl, self.yt_se, self.g4, self.a1, mt.iRk, self.cphase,
                    self._aslot(2), self._aslot(3), None)
         self._gemm(self.YTh, self.YTl, self.yt_se, self.a1, self.gam, mt.Rk, None,
                    self._aslot(3), None, None)
-        call("fp_adj_plasma_c64", ptr(un), ptr(rho_n), ptr(self.grho), ptr(self.gam),
-             ctypes.byref(t.plasma_params), ptr(t.W_lut), ptr(mt.tw), self._aslot(0), st)
+        call("fp_adj_plasma_c64", ptr(un), ptr(rho_n), ptr(rhos_n), ptr(self.grho),
+             ptr(self.grho_s), ptr(self.gam), ctypes.byref(t.plasma_params), ptr(t.W_lut),
+             ptr(t.Ws_lut if self.ste else None), ptr(mt.tw), self._aslot(0), st)
 
     def backward(self, dF):
         """Gradient w.r.t. the input field (time domain, complex64) of a loss whose

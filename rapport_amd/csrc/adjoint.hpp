@@ -13,13 +13,17 @@
 //   A(v) = v exp(-alpha(|v|^2, rho) dz/2), recomputes the forward pass in the row
 //   (the RK4 stage inputs go to a caller scratch of 4 planes), then applies the
 //   adjoints in reverse: A at v3, the four RK4 stages (each the adjoint of the
-//   split() right-hand side at that stage input, which also accumulates dL/drho),
-//   A at v1. STE off (the STE index term is not differentiated).
+//   split() right-hand side at that stage input, which also accumulates dL/drho and,
+//   with the STE index term i ste_pref rho_s u, dL/drho_s), A at v1.
 // adj_plasma_kernel: dL/du from dL/drho through the rate-equation recurrence
 //   (kernels.py:36-106, STE off) -- the backward linear recurrence
 //   lambda_k = g_k + (d n_{k+1}/d n_k) lambda_{k+1} solved as a parallel scan of
 //   affine maps over per-thread chunks -- and adds FFT_t(2 invE2 dL/dI u) to the
 //   row of the spectral gradient.
+// adj_plasma_ste_kernel: the same with STE on: the state is (n_e, n_s), each step's
+//   Jacobian is 2 x 2 (coupled exact-exponential updates, clamps at 0, the
+//   n_e + n_s <= n_a rescaling), the chunk maps are 2 x 2 affine maps, and dL/dI also
+//   collects the STE rate's LUT derivative.
 #pragma once
 #include "common.hpp"
 #include "fft.hpp"
@@ -87,11 +91,13 @@ template <int NT> struct AdjSmem {
   float sr[NT];                  // rho row
 };
 
-// mu = (d rhs / d y)^H lam at stage input y; grho += (d rhs / d rho)^T lam.
-template <int NT, int TPB>
+// mu = (d rhs / d y)^H lam at stage input y; grho += (d rhs / d rho)^T lam and, when
+// RS (the STE index term is on; rhos_row = the row's rho_s), grhos += (d rhs / d rho_s)^T lam.
+template <int NT, int TPB, bool RS>
 FP_DEV void rhs_adj_row(const c32 (&y)[NT / TPB], const c32 (&lam)[NT / TPB], c32 (&mu)[NT / TPB],
-                        float (&grho)[NT / TPB], AdjSmem<NT>& sm, const fp_nl_params& p,
-                        const fp_nl_tables_f32& t) {
+                        float (&grho)[NT / TPB], float (&grhos)[NT / TPB],
+                        const double* __restrict__ rhos_row, AdjSmem<NT>& sm,
+                        const fp_nl_params& p, const fp_nl_tables_f32& t) {
   constexpr int E = NT / TPB;
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
@@ -173,8 +179,12 @@ FP_DEV void rhs_adj_row(const c32 (&y)[NT / TPB], const c32 (&lam)[NT / TPB], c3
       grho[s] += pr * re;
     }
   }
-  if (p.pc_re != 0.0 || p.pc_im != 0.0) {
+  // extra = (q rho + i ste_pref rho_s) y: mu += conj(q rho + i qs) gc, qs = ste_pref rho_s
+  // rounded to float as the forward rounds it; d/drho_s = i ste_pref y.
+  const bool ste_on = RS && p.ste_pref != 0.0;
+  if (p.pc_re != 0.0 || p.pc_im != 0.0 || ste_on) {
     const c32 q = mk<float>((float)(-p.plasma_pref * p.pc_re), (float)(-p.plasma_pref * p.pc_im));
+    const float sp = (float)p.ste_pref;
 #pragma unroll
     for (int s = 0; s < E; ++s) buf[s] = t.mC[tid + s * TPB] * sm.sG[tid + s * TPB];
     row_fft<float, NT, TPB, true>(buf, sm.lds, tw);
@@ -185,6 +195,11 @@ FP_DEV void rhs_adj_row(const c32 (&y)[NT / TPB], const c32 (&lam)[NT / TPB], c3
       mu[s] = mu[s] + r * (conjx(q) * gc);
       const c32 qy = q * y[s];
       grho[s] += gc.x * qy.x + gc.y * qy.y;
+      if (ste_on) {
+        const float qs = (float)(p.ste_pref * rhos_row[tid + s * TPB]);
+        mu[s] = mu[s] + mk<float>(qs * gc.y, -qs * gc.x);          // (-i qs) gc
+        grhos[s] += sp * (gc.y * y[s].x - gc.x * y[s].y);          // Re(conj(gc) i sp y)
+      }
     }
   }
 #pragma unroll
@@ -200,15 +215,17 @@ FP_DEV void load_plane(const float* base, size_t plane, int k, size_t row, c32 (
 }
 
 // Inputs: v1s = FFT_t of the field after the first half linear step (G2 output),
-// rho (FP64), g4s = gradient w.r.t. the input of the second half linear step
-// (spectral). Outputs: g1s = gradient w.r.t. v1 (spectral), grho_out (float),
-// cmax_out = max |re|, |im| of g1s.
-template <int NT, int TPB>
+// rho (FP64), rhos (FP64 rho_s, read when RS), g4s = gradient w.r.t. the input of the
+// second half linear step (spectral). Outputs: g1s = gradient w.r.t. v1 (spectral),
+// grho_out (float), grhos_out (float, written when RS), cmax_out = max |re|, |im| of g1s.
+template <int NT, int TPB, bool RS>
 __global__ __launch_bounds__(TPB) void adj_nl_kernel(const float* __restrict__ v1s,
                                                      const double* __restrict__ rho,
+                                                     const double* __restrict__ rhos,
                                                      const float* __restrict__ g4s,
                                                      float* __restrict__ g1s,
                                                      float* __restrict__ grho_out,
+                                                     float* __restrict__ grhos_out,
                                                      float* __restrict__ scratch, fp_nl_params p,
                                                      fp_nl_tables_f32 t,
                                                      unsigned* __restrict__ cmax_out) {
@@ -220,6 +237,7 @@ __global__ __launch_bounds__(TPB) void adj_nl_kernel(const float* __restrict__ v
   const c32* tw = reinterpret_cast<const c32*>(t.tw);
   const float invN = 1.0f / (float)NT;
   const float dz = (float)p.dz, hdz = -0.5f * dz;
+  const double* rhos_row = RS ? rhos + row * NT : nullptr;
 #pragma unroll
   for (int s = 0; s < E; ++s) sm.sr[tid + s * TPB] = (float)rho[row * NT + tid + s * TPB];
   c32 v[E];
@@ -242,7 +260,7 @@ __global__ __launch_bounds__(TPB) void adj_nl_kernel(const float* __restrict__ v
     c32 acc[E], k[E];
 #pragma unroll 1
     for (int stg = 0; stg < 4; ++stg) {
-      fpnl32::split_row<NT, TPB, false>(v, sm.sr, nullptr, k, nullptr, p, t, sm.lds);
+      fpnl32::split_row<NT, TPB, RS>(v, sm.sr, rhos_row, k, nullptr, p, t, sm.lds);
       const float w = (stg == 0 || stg == 3) ? 1.0f : 2.0f;
       const float cdz = (stg < 2) ? 0.5f * dz : dz;
 #pragma unroll
@@ -258,9 +276,10 @@ __global__ __launch_bounds__(TPB) void adj_nl_kernel(const float* __restrict__ v
   c32 g[E];
   fpnl32::load_row<NT, TPB>(g4s, row, g);
   row_fft<float, NT, TPB, true>(g, sm.lds, tw);
-  float grho[E];
+  float grho[E], grhos[E];
 #pragma unroll
   for (int s = 0; s < E; ++s) {
+    grhos[s] = 0.0f;
     g[s] = invN * g[s];
     const float a2 = v[s].x * v[s].x + v[s].y * v[s].y;
     const float al = alpha_d(a2, sm.sr[tid + s * TPB], p, t, ph, pa, pr, da, dr);
@@ -295,7 +314,7 @@ __global__ __launch_bounds__(TPB) void adj_nl_kernel(const float* __restrict__ v
         y[s] = expf(hdz * al) * y[s];
       }
     }
-    rhs_adj_row<NT, TPB>(y, lam, mu, grho, sm, p, t);
+    rhs_adj_row<NT, TPB, RS>(y, lam, mu, grho, grhos, rhos_row, sm, p, t);
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = v[s] + mu[s];
   }
@@ -311,6 +330,7 @@ __global__ __launch_bounds__(TPB) void adj_nl_kernel(const float* __restrict__ v
     g[s] = mk<float>(e * g2.x + 2.0f * sc * da * y[s].x, e * g2.y + 2.0f * sc * da * y[s].y);
     grho[s] += sc * dr;
     grho_out[row * NT + tid + s * TPB] = nan0(grho[s]);
+    if (RS) grhos_out[row * NT + tid + s * TPB] = nan0(grhos[s]);
   }
   row_fft<float, NT, TPB, false>(g, sm.lds, tw);
   fpnl32::store_row<NT, TPB>(g1s, row, g);
@@ -473,6 +493,232 @@ __global__ __launch_bounds__(TPB) void adj_plasma_kernel(const float* __restrict
     for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
     __syncthreads();
     float* sred = reinterpret_cast<float*>(sm.sA);
+    if ((tid & 63) == 0) sred[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < TPB / 64; ++w) m = fmaxf(m, sred[w]);
+      atomicMax(cmax_out, __float_as_uint(m));
+    }
+  }
+}
+
+// One step of the carrier recurrence with STE (fpplasma::rate_step<true>) linearised
+// for the backward sweep: lambda_k = g_k + M lambda_{k+1}, lambda = dL/d(n_e, n_s),
+// and dL/dI_avg = ie le + is ls, dL/dW_avg = we le + ws ls, dL/dWs_avg = se le + ss ls
+// for lambda_{k+1} = (le, ls). a = E(n_e; S_e, L_e) and b = E(n_s; S_s, L_s) are the
+// clamped exact-exponential updates; where a + b > n_a the step returns (a, b) n_a /
+// (a + b), whose Jacobian n_a / (a + b)^2 [[b, -a], [-b, a]] is applied first.
+struct StepAdj {
+  double m00, m01, m10, m11;
+  double ie, is, we, ws, se, ss;
+};
+
+FP_DEV StepAdj ste_step_adj(double ne, double ns, double Ia, double Wa, double Wsa,
+                            const fp_plasma_params& p) {
+  const double na = p.na, dt = p.dt, bg = p.beta_g, bs = p.beta_s, itr = p.inv_tau_r;
+  const double dl = 1.0 - ne / na;
+  const double depl = dl < 0.0 ? 0.0 : (dl > 1.0 ? 1.0 : dl);
+  const double dd = (dl > 0.0 && dl < 1.0) ? -1.0 / na : 0.0;
+  const double wst = Wsa + bs * Ia * ne;
+  const double Se = Wa * depl + wst * (ns / na);
+  const double Le = bg * Ia * depl - itr;
+  const fpplasma::Step sa = fpplasma::exp_step(ne, Se, Le, dt);
+  const bool oka = fp_isfinite(sa.xnew) && sa.xnew >= 0.0;
+  const double a = oka ? sa.xnew : 0.0;
+  const double aS = dt * sa.phi1, aL = dt * sa.E * ne + Se * dt * dt * sa.dphi1;
+  const double a_ne = sa.E + aS * (Wa * dd + bs * Ia * (ns / na)) + aL * bg * Ia * dd;
+  const double a_ns = aS * wst / na;
+  const double a_I = aS * bs * ne * (ns / na) + aL * bg * depl;
+  const double a_W = aS * depl;
+  const double a_Ws = aS * (ns / na);
+  const double Ss = itr * ne, Ls = -wst / na - p.inv_tau_ste;
+  const fpplasma::Step sb = fpplasma::exp_step(ns, Ss, Ls, dt);
+  const bool okb = fp_isfinite(sb.xnew) && sb.xnew >= 0.0;
+  const double b = okb ? sb.xnew : 0.0;
+  const double bS = dt * sb.phi1, bL = dt * sb.E * ns + Ss * dt * dt * sb.dphi1;
+  const double b_ne = bS * itr - bL * bs * Ia / na;
+  const double b_ns = sb.E;
+  const double b_I = -bL * bs * ne / na;
+  const double b_Ws = -bL / na;
+  // mu_a = rae le + ras ls, mu_b = rbe le + rbs ls: dL/d(a, b)
+  double rae = 1.0, ras = 0.0, rbe = 0.0, rbs = 1.0;
+  const double tot = a + b;
+  if (tot > na) {
+    const double f = na / (tot * tot);
+    rae = f * b; ras = -f * b; rbe = -f * a; rbs = f * a;
+  }
+  if (!oka) { rae = 0.0; ras = 0.0; }
+  if (!okb) { rbe = 0.0; rbs = 0.0; }
+  StepAdj s;
+  s.m00 = rae * a_ne + rbe * b_ne;
+  s.m01 = ras * a_ne + rbs * b_ne;
+  s.m10 = rae * a_ns + rbe * b_ns;
+  s.m11 = ras * a_ns + rbs * b_ns;
+  s.ie = rae * a_I + rbe * b_I;
+  s.is = ras * a_I + rbs * b_I;
+  s.we = rae * a_W;
+  s.ws = ras * a_W;
+  s.se = rae * a_Ws + rbe * b_Ws;
+  s.ss = ras * a_Ws + rbs * b_Ws;
+  return s;
+}
+
+template <int NT, int TPB>
+struct AdjPlasmaSteSmem {
+  double sI[NT];
+  double sGI[NT];
+  double sM[6][TPB];     // chunk maps (A row-major, B) for the suffix scan
+  c32 lds[LdsSize<NT>::value];
+};
+
+// us, rho, rhos: the spectral state the forward scan read and its densities; grho,
+// grhos: dL/drho, dL/drho_s; gs: spectral gradient row, += FFT_t of 2 invE2 (dL/dI) u.
+// Each thread owns NT/TPB consecutive samples. Pass 1 composes its chunk's affine map
+// lambda_{k0} = A lambda_{k0+E} + B backwards, the suffix scan over threads gives each
+// chunk its incoming lambda_{k0+E}, and pass 2 re-linearises every step (the LUT rates
+// and step Jacobians are recomputed rather than kept: 10 doubles a step would not fit
+// in registers) and accumulates dL/dI in LDS.
+template <int NT, int TPB>
+__global__ __launch_bounds__(TPB) void adj_plasma_ste_kernel(
+    const float* __restrict__ us, const double* __restrict__ rho, const double* __restrict__ rhos,
+    const float* __restrict__ grho, const float* __restrict__ grhos, float* __restrict__ gs,
+    fp_plasma_params p, const double* __restrict__ Wg, const double* __restrict__ Wsg,
+    const float* __restrict__ tw_, unsigned* __restrict__ cmax_out) {
+  constexpr int E = NT / TPB;
+  __shared__ AdjPlasmaSteSmem<NT, TPB> sm;
+  const size_t row = blockIdx.x;
+  const int tid = threadIdx.x;
+  const c32* tw = reinterpret_cast<const c32*>(tw_);
+  c32 u[E];
+  fpnl32::load_row<NT, TPB>(us, row, u);
+  row_fft<float, NT, TPB, true>(u, sm.lds, tw);
+  const float invN = 1.0f / (float)NT;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    u[s] = invN * u[s];
+    const double x = (double)u[s].x, y = (double)u[s].y;
+    sm.sI[tid + s * TPB] = (x * x + y * y) * p.field_to_I;
+  }
+  __syncthreads();
+  const int k0 = tid * E;
+  const double* nrow = rho + row * NT;
+  const double* srow = rhos + row * NT;
+  const float* ge = grho + row * NT;
+  const float* gsr = grhos + row * NT;
+  // rates at the chunk's right neighbour k0 + E (the next chunk's first sample)
+  double Wr = 0.0, dWr = 0.0, Wsr = 0.0, dWsr = 0.0;
+  if (k0 + E < NT) {
+    Wr = lut_rate_d(sm.sI[k0 + E], Wg, p, dWr);
+    Wsr = lut_rate_d(sm.sI[k0 + E], Wsg, p, dWsr);
+  }
+  // pass 1: chunk map
+  double A00 = 1.0, A01 = 0.0, A10 = 0.0, A11 = 1.0, B0 = 0.0, B1 = 0.0;
+  {
+    double W1 = Wr, Ws1 = Wsr;
+#pragma unroll 1
+    for (int j = E - 1; j >= 0; --j) {
+      const int k = k0 + j;
+      double d0;
+      const double W0 = lut_rate_d(sm.sI[k], Wg, p, d0);
+      const double Ws0 = lut_rate_d(sm.sI[k], Wsg, p, d0);
+      if (k < NT - 1) {
+        const StepAdj st = ste_step_adj(nrow[k], srow[k], 0.5 * (sm.sI[k] + sm.sI[k + 1]),
+                                        0.5 * (W0 + W1), 0.5 * (Ws0 + Ws1), p);
+        const double n00 = st.m00 * A00 + st.m01 * A10, n01 = st.m00 * A01 + st.m01 * A11;
+        const double n10 = st.m10 * A00 + st.m11 * A10, n11 = st.m10 * A01 + st.m11 * A11;
+        const double c0 = st.m00 * B0 + st.m01 * B1 + (double)ge[k];
+        const double c1 = st.m10 * B0 + st.m11 * B1 + (double)gsr[k];
+        A00 = n00; A01 = n01; A10 = n10; A11 = n11; B0 = c0; B1 = c1;
+      } else {                                   // last sample: lambda_{NT-1} = g_{NT-1}
+        A00 = A01 = A10 = A11 = 0.0;
+        B0 = (double)ge[k];
+        B1 = (double)gsr[k];
+      }
+      W1 = W0;
+      Ws1 = Ws0;
+    }
+  }
+  // suffix composition F_t o F_{t+1} o ... : A = A_t A', B = A_t B' + B_t
+  sm.sM[0][tid] = A00; sm.sM[1][tid] = A01; sm.sM[2][tid] = A10; sm.sM[3][tid] = A11;
+  sm.sM[4][tid] = B0; sm.sM[5][tid] = B1;
+  __syncthreads();
+  for (int off = 1; off < TPB; off <<= 1) {
+    const bool has = tid + off < TPB;
+    double o00 = 1.0, o01 = 0.0, o10 = 0.0, o11 = 1.0, ob0 = 0.0, ob1 = 0.0;
+    if (has) {
+      o00 = sm.sM[0][tid + off]; o01 = sm.sM[1][tid + off];
+      o10 = sm.sM[2][tid + off]; o11 = sm.sM[3][tid + off];
+      ob0 = sm.sM[4][tid + off]; ob1 = sm.sM[5][tid + off];
+    }
+    __syncthreads();
+    if (has) {
+      const double c0 = A00 * ob0 + A01 * ob1 + B0, c1 = A10 * ob0 + A11 * ob1 + B1;
+      const double n00 = A00 * o00 + A01 * o10, n01 = A00 * o01 + A01 * o11;
+      const double n10 = A10 * o00 + A11 * o10, n11 = A10 * o01 + A11 * o11;
+      A00 = n00; A01 = n01; A10 = n10; A11 = n11; B0 = c0; B1 = c1;
+    }
+    sm.sM[0][tid] = A00; sm.sM[1][tid] = A01; sm.sM[2][tid] = A10; sm.sM[3][tid] = A11;
+    sm.sM[4][tid] = B0; sm.sM[5][tid] = B1;
+    __syncthreads();
+  }
+  // lambda_{k0+E}: the suffix of the next chunk applied to lambda_NT = 0
+  double le = (tid + 1 < TPB) ? sm.sM[4][tid + 1] : 0.0;
+  double ls = (tid + 1 < TPB) ? sm.sM[5][tid + 1] : 0.0;
+  // pass 2: dL/dI of the chunk's samples (sGI[k0 .. k0+E-1] are this thread's)
+#pragma unroll 1
+  for (int j = 0; j < E; ++j) sm.sGI[k0 + j] = 0.0;
+  double carry = 0.0;                            // -> dL/dI at k0 + E
+  {
+    double W1 = Wr, dW1 = dWr, Ws1 = Wsr, dWs1 = dWsr;
+#pragma unroll 1
+    for (int j = E - 1; j >= 0; --j) {
+      const int k = k0 + j;
+      double dW0, dWs0;
+      const double W0 = lut_rate_d(sm.sI[k], Wg, p, dW0);
+      const double Ws0 = lut_rate_d(sm.sI[k], Wsg, p, dWs0);
+      if (k < NT - 1) {
+        const StepAdj st = ste_step_adj(nrow[k], srow[k], 0.5 * (sm.sI[k] + sm.sI[k + 1]),
+                                        0.5 * (W0 + W1), 0.5 * (Ws0 + Ws1), p);
+        const double gIa = st.ie * le + st.is * ls;
+        const double gWa = st.we * le + st.ws * ls;
+        const double gWsa = st.se * le + st.ss * ls;
+        sm.sGI[k] += 0.5 * (gIa + gWa * dW0 + gWsa * dWs0);
+        const double right = 0.5 * (gIa + gWa * dW1 + gWsa * dWs1);
+        if (j + 1 < E) sm.sGI[k + 1] += right;
+        else carry = right;
+        const double nle = (double)ge[k] + st.m00 * le + st.m01 * ls;
+        const double nls = (double)gsr[k] + st.m10 * le + st.m11 * ls;
+        le = nle;
+        ls = nls;
+      } else {
+        le = (double)ge[k];
+        ls = (double)gsr[k];
+      }
+      W1 = W0; dW1 = dW0; Ws1 = Ws0; dWs1 = dWs0;
+    }
+  }
+  __syncthreads();
+  if (k0 + E < NT) sm.sGI[k0 + E] += carry;
+  __syncthreads();
+  const double f2 = 2.0 * p.field_to_I;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const float c = (float)(f2 * sm.sGI[tid + s * TPB]);
+    u[s] = c * u[s];
+  }
+  row_fft<float, NT, TPB, false>(u, sm.lds, tw);
+  c32* grow_s = reinterpret_cast<c32*>(gs) + row * NT;
+  float m = 0.0f;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const c32 r = grow_s[tid + s * TPB] + u[s];
+    grow_s[tid + s * TPB] = r;
+    m = fmaxf(m, fmaxf(fabsf(r.x), fabsf(r.y)));
+  }
+  if (cmax_out) {
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    __syncthreads();
+    float* sred = reinterpret_cast<float*>(&sm.sM[0][0]);
     if ((tid & 63) == 0) sred[tid >> 6] = m;
     __syncthreads();
     if (tid == 0) {

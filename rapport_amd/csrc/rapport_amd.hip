@@ -1357,17 +1357,24 @@ int fp_batch_launches(const fp_batch_plan* plan) {
 }
 
 
-int fp_adj_nl_step_c64(const float* v1s, const double* rho, const float* g4s, float* g1s,
-                       float* grho, float* scratch, const fp_nl_params* p,
-                       const fp_nl_tables_f32* t, unsigned* cmax_out, void* stream) {
+int fp_adj_nl_step_c64(const float* v1s, const double* rho, const double* rho_s,
+                       const float* g4s, float* g1s, float* grho, float* grho_s, float* scratch,
+                       const fp_nl_params* p, const fp_nl_tables_f32* t, unsigned* cmax_out,
+                       void* stream) {
   if (!v1s || !rho || !g4s || !g1s || !grho || !scratch || !p || !t)
     return fail(FP_E_ARG, "fp_adj_nl_step_c64: null pointer");
+  if (rho_s && !grho_s) return fail(FP_E_ARG, "fp_adj_nl_step_c64: rho_s given without grho_s");
   if (p->Nr <= 0) return fail(FP_E_ARG, "fp_adj_nl_step_c64: bad Nr");
   if (pcoef_misaligned(t)) return fail(FP_E_ARG, "fp_adj_nl_step_c64: pcoef_s must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define FP_ADJNL(NT)                                                                           \
-  hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 4>), dim3(p->Nr), dim3(NT / 4), 0, st, v1s, \
-                     rho, g4s, g1s, grho, scratch, *p, *t, cmax_out);                          \
+  if (rho_s)                                                                                   \
+    hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 4, true>), dim3(p->Nr), dim3(NT / 4), 0, \
+                       st, v1s, rho, rho_s, g4s, g1s, grho, grho_s, scratch, *p, *t, cmax_out); \
+  else                                                                                         \
+    hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 4, false>), dim3(p->Nr), dim3(NT / 4),   \
+                       0, st, v1s, rho, nullptr, g4s, g1s, grho, nullptr, scratch, *p, *t,     \
+                       cmax_out);                                                              \
   return check_launch("adj_nl_step_c64");
   switch (p->Nt) {
     case 512: { FP_ADJNL(512) }
@@ -1379,17 +1386,23 @@ int fp_adj_nl_step_c64(const float* v1s, const double* rho, const float* g4s, fl
 #undef FP_ADJNL
 }
 
-int fp_adj_plasma_c64(const float* us, const double* rho, const float* grho, float* gs,
-                      const fp_plasma_params* p, const double* W_lut, const float* tw,
+int fp_adj_plasma_c64(const float* us, const double* rho, const double* rho_s, const float* grho,
+                      const float* grho_s, float* gs, const fp_plasma_params* p,
+                      const double* W_lut, const double* Ws_lut, const float* tw,
                       unsigned* cmax_out, void* stream) {
   if (!us || !rho || !grho || !gs || !p || !W_lut || !tw)
     return fail(FP_E_ARG, "fp_adj_plasma_c64: null pointer");
-  if (p->enable_ste) return fail(FP_E_ARG, "fp_adj_plasma_c64: the adjoint covers STE off");
+  if (p->enable_ste && (!rho_s || !grho_s || !Ws_lut))
+    return fail(FP_E_ARG, "fp_adj_plasma_c64: STE on needs rho_s, grho_s and Ws_lut");
   if (p->Nr <= 0 || p->table_size < 2) return fail(FP_E_ARG, "fp_adj_plasma_c64: bad Nr/table_size");
   hipStream_t st = S(stream);
-#define FP_ADJPL(NT)                                                                         \
-  hipLaunchKernelGGL((fpadj::adj_plasma_kernel<NT, 256>), dim3(p->Nr), dim3(256), 0, st, us, \
-                     rho, grho, gs, *p, W_lut, tw, cmax_out);                                \
+#define FP_ADJPL(NT)                                                                            \
+  if (p->enable_ste)                                                                            \
+    hipLaunchKernelGGL((fpadj::adj_plasma_ste_kernel<NT, 256>), dim3(p->Nr), dim3(256), 0, st,  \
+                       us, rho, rho_s, grho, grho_s, gs, *p, W_lut, Ws_lut, tw, cmax_out);      \
+  else                                                                                          \
+    hipLaunchKernelGGL((fpadj::adj_plasma_kernel<NT, 256>), dim3(p->Nr), dim3(256), 0, st, us,  \
+                       rho, grho, gs, *p, W_lut, tw, cmax_out);                                 \
   return check_launch("adj_plasma_c64");
   switch (p->Nt) {
     case 512: { FP_ADJPL(512) }

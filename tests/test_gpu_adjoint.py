@@ -2,13 +2,15 @@
 (oracle/adjoint_numpy.py, itself pinned by finite differences on the CPU).
 
 Case: Nr x Nt = 128 x 512, w0 = 30 um, 15 uJ, dz = 3 um -- Kerr + Raman,
-photo-ionisation loss and plasma all active (rho > 1e14 cm^-3). Bars, FP32 kernels
-against FP64 (stated per output):
-* nonlinear-step adjoint (g4 -> g1, dL/drho): rel-L2 <= 1e-5;
-* carrier-scan adjoint: rel-L2 <= 1e-5;
+photo-ionisation loss and plasma all active (rho > 1e14 cm^-3); and the same with STE
+on (tau_r 60 fs, tau_ste 150 fs, 20 uJ: rho and rho_s ~3e19 cm^-3, the STE index term
+~0.02 rad per step). Bars, FP32 kernels against FP64 (stated per output):
+* nonlinear-step adjoint (g4 -> g1, dL/drho, dL/drho_s): rel-L2 <= 1e-5;
+* carrier-scan adjoint (scalar scan; 2 x 2 scan with STE): rel-L2 <= 1e-5;
 * one whole adjoint step (GEMMs + both kernels): rel-L2 <= 2e-5;
 * gradient of the fluence loss w.r.t. 16 Laguerre-Gauss coefficients after 8
-  steps (mixed forward vs FP64 forward and adjoints): rel-L2 <= 1e-4, loss 1e-4;
+  steps (mixed forward vs FP64 forward and adjoints; STE off, and STE on at 15 uJ):
+  rel-L2 <= 1e-4, loss 1e-4;
 * the torch.autograd.Function gives the engine's gradient, and a short L-BFGS fit
   lowers the loss by at least 10x.
 """
@@ -31,21 +33,45 @@ def _announce(msg):
     sys.stdout.flush()
 
 
-def _kw(nz=8):
-    return dict(MATERIAL, nz=nz, Nt=512, N=129, R_factor=6.0, begin=0.0, end=nz * 3e-6,
-                t_window=400e-15, energy_uJ=15.0, w0=30e-6, save_stride=10 ** 6, rho_t_stride=0,
-                write_output=False, verbose=False)
+STE = dict(enable_ste=True, tau_r=60e-15, tau_ste=150e-15, energy_uJ=20.0)
+
+
+def _kw(nz=8, **over):
+    kw = dict(MATERIAL, nz=nz, Nt=512, N=129, R_factor=6.0, begin=0.0, end=nz * 3e-6,
+              t_window=400e-15, energy_uJ=15.0, w0=30e-6, save_stride=10 ** 6, rho_t_stride=0,
+              write_output=False, verbose=False)
+    kw.update(over)
+    return kw
+
+
+def _make_pair(**over):
+    import rapport_amd as R
+    from rapport_amd.adjoint import AdjointEngine
+    env_o = O.envelope_lens_noise(noise=0.05, seed=7)
+    oa = A.Adjoint(O.Integrator(O.Config(**_kw(**over)), envelope=env_o))
+    it = R.Integrator(R.Config(**_kw(**over), precision="mixed"),
+                      envelope=R.envelope_lens_noise(noise=0.05, seed=7))
+    return oa, AdjointEngine(it)
 
 
 @pytest.fixture(scope="module")
 def pair():
-    import rapport_amd as R
-    from rapport_amd.adjoint import AdjointEngine
-    env_o = O.envelope_lens_noise(noise=0.05, seed=7)
-    oa = A.Adjoint(O.Integrator(O.Config(**_kw()), envelope=env_o))
-    it = R.Integrator(R.Config(**_kw(), precision="mixed"),
-                      envelope=R.envelope_lens_noise(noise=0.05, seed=7))
-    return oa, AdjointEngine(it)
+    return _make_pair()
+
+
+@pytest.fixture(scope="module")
+def pair_ste():
+    return _make_pair(**STE)
+
+
+@pytest.fixture(scope="module")
+def pair_ste_lg():
+    """STE on at 15 uJ: the Laguerre-Gauss inputs are more peaked than the lens-noise
+    envelope (rho_s ~6e19 cm^-3 in the first step); at 20 uJ their 8-step run diverges."""
+    return _make_pair(**dict(STE, energy_uJ=15.0))
+
+
+BOTH = pytest.mark.parametrize("which", ["pair", "pair_ste"])
 
 
 def _c64(a):
@@ -56,68 +82,88 @@ def _spec(a):
     return np.fft.fft(a, axis=1)
 
 
-def test_adj_nl_step_kernel(pair):
+@BOTH
+def test_adj_nl_step_kernel(which, request):
     import ctypes
     from rapport_amd._lib import call, ptr, stream_ptr
-    oa, eng = pair
-    _announce("adjoint nonlinear step kernel vs oracle")
+    oa, eng = request.getfixturevalue(which)
+    ste = eng.ste
+    _announce(f"adjoint nonlinear step kernel vs oracle (STE {'on' if ste else 'off'})")
     u = oa.step(oa.it.u)                       # a state after one step
     u = u.astype(np.complex64).astype(np.complex128)
-    rho = oa.plasma(u)
-    assert rho.max() > 1e14
+    rho, rho_s = oa.plasma(u)
+    assert rho.max() > 1e14 and (not ste or rho_s.max() > 1e19)
     v1 = oa.lin.half_linear(u, oa.dz).astype(np.complex64).astype(np.complex128)
     rng = np.random.default_rng(1)
     g4 = (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)) * np.abs(v1) / np.abs(v1).max()
     v2 = oa.alpha_half(v1, rho)
-    v3, ys = oa.rk4(v2, rho)
+    v3, ys = oa.rk4(v2, rho, rho_s)
     g3, gr1 = oa.alpha_half_adj(v3, rho, g4)
-    g2, gr2 = oa.rk4_adj(ys, rho, g3)
+    g2, gr2, grs = oa.rk4_adj(ys, rho, rho_s, g3)
     g1, gr3 = oa.alpha_half_adj(v1, rho, g2)
     v1s, g4s = _c64(_spec(v1)), _c64(_spec(g4))
     rho_d = torch.from_numpy(rho).cuda()
+    rhos_d = torch.from_numpy(rho_s).cuda() if ste else None
     g1s = torch.empty_like(g4s)
     grho = torch.empty(u.shape, dtype=torch.float32, device="cuda")
+    grhos = torch.empty(u.shape, dtype=torch.float32, device="cuda") if ste else None
     mt = eng.mx.m
-    call("fp_adj_nl_step_c64", ptr(v1s), ptr(rho_d), ptr(g4s), ptr(g1s), ptr(grho),
-         ptr(eng.scratch), ctypes.byref(eng.nlp), ctypes.byref(mt.tables), None, stream_ptr())
+    call("fp_adj_nl_step_c64", ptr(v1s), ptr(rho_d), ptr(rhos_d), ptr(g4s), ptr(g1s), ptr(grho),
+         ptr(grhos), ptr(eng.scratch), ctypes.byref(eng.nlp), ctypes.byref(mt.tables), None,
+         stream_ptr())
     torch.cuda.synchronize()
     e1 = rel_l2(g1s.cpu().numpy(), _spec(g1))
     e2 = rel_l2(grho.cpu().numpy(), gr1 + gr2 + gr3)
-    _announce(f"  g1 rel-L2 {e1:.2e}, grho rel-L2 {e2:.2e}")
-    assert e1 <= 1e-5 and e2 <= 1e-5
+    e3 = rel_l2(grhos.cpu().numpy(), grs) if ste else 0.0
+    _announce(f"  g1 rel-L2 {e1:.2e}, grho rel-L2 {e2:.2e}, grho_s rel-L2 {e3:.2e}")
+    assert e1 <= 1e-5 and e2 <= 1e-5 and e3 <= 1e-5
+    if ste:
+        assert np.abs(grs).max() > 0
 
 
-def test_adj_plasma_kernel(pair):
+@BOTH
+def test_adj_plasma_kernel(which, request):
     import ctypes
     from rapport_amd._lib import call, ptr, stream_ptr
-    oa, eng = pair
-    _announce("adjoint carrier scan kernel vs oracle")
+    oa, eng = request.getfixturevalue(which)
+    ste = eng.ste
+    _announce(f"adjoint carrier scan kernel vs oracle (STE {'on' if ste else 'off'})")
     u = oa.step(oa.it.u).astype(np.complex64).astype(np.complex128)
     rng = np.random.default_rng(2)
     us = _c64(_spec(u))
     rho_d = torch.zeros(u.shape, dtype=torch.float64, device="cuda")
+    rhos_d = torch.zeros(u.shape, dtype=torch.float64, device="cuda") if ste else None
     it = eng.it
-    call("fp_plasma_scan_spec_c64", ptr(us), ptr(rho_d), None, None,
+    call("fp_plasma_scan_spec_c64", ptr(us), ptr(rho_d), ptr(rhos_d), None,
          ctypes.byref(it.t.plasma_params), ptr(it.t.W_lut), ptr(it.t.Ws_lut), ptr(eng.mx.m.tw),
          None, ptr(it.t.status), stream_ptr())
     torch.cuda.synchronize()
-    rho = oa.plasma(u)
+    rho, rho_s = oa.plasma(u)
     assert rel_l2(rho_d.cpu().numpy(), rho) < 1e-6 and rho.max() > 1e14
+    if ste:
+        assert rel_l2(rhos_d.cpu().numpy(), rho_s) < 1e-6 and rho_s.max() > 1e19
     gr = rng.standard_normal(u.shape).astype(np.float32)
+    grs = rng.standard_normal(u.shape).astype(np.float32) if ste else np.zeros(u.shape, np.float32)
     gs0 = (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)) * 1e-3
-    expect = gs0 + _spec(oa.plasma_adj(u, rho, gr.astype(np.float64)))
+    expect = gs0 + _spec(oa.plasma_adj(u, rho, rho_s, gr.astype(np.float64), grs.astype(np.float64)))
     gs = _c64(gs0)
-    call("fp_adj_plasma_c64", ptr(us), ptr(rho_d), ptr(torch.from_numpy(gr).cuda()), ptr(gs),
-         ctypes.byref(it.t.plasma_params), ptr(it.t.W_lut), ptr(eng.mx.m.tw), None, stream_ptr())
+    # device copies held in names: a temporary freed inside the argument list would hand
+    # its block to the next allocation, and both pointers would see the second array
+    gr_d = torch.from_numpy(gr).cuda()
+    grs_d = torch.from_numpy(grs).cuda() if ste else None
+    call("fp_adj_plasma_c64", ptr(us), ptr(rho_d), ptr(rhos_d), ptr(gr_d), ptr(grs_d), ptr(gs),
+         ctypes.byref(it.t.plasma_params), ptr(it.t.W_lut), ptr(it.t.Ws_lut if ste else None),
+         ptr(eng.mx.m.tw), None, stream_ptr())
     torch.cuda.synchronize()
     e = rel_l2(gs.cpu().numpy(), expect)
     _announce(f"  rel-L2 {e:.2e}")
     assert e <= 1e-5
 
 
-def test_adjoint_step(pair):
-    oa, eng = pair
-    _announce("one adjoint step (GEMMs + kernels) vs oracle")
+@BOTH
+def test_adjoint_step(which, request):
+    oa, eng = request.getfixturevalue(which)
+    _announce(f"one adjoint step (GEMMs + kernels) vs oracle (STE {'on' if eng.ste else 'off'})")
     u = oa.step(oa.it.u).astype(np.complex64).astype(np.complex128)
     rng = np.random.default_rng(3)
     gn = (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)) * np.abs(u) / np.abs(u).max()
@@ -143,11 +189,13 @@ def _lg_setup(P, seed):
     return theta, hidden
 
 
-def test_lg_gradient_vs_oracle(pair):
+@pytest.mark.parametrize("which", ["pair", "pair_ste_lg"])
+def test_lg_gradient_vs_oracle(which, request):
     from rapport_amd import adjoint as AD
-    oa, eng = pair
+    oa, eng = request.getfixturevalue(which)
     n, P = 8, 8
-    _announce(f"LG gradient ({2 * P} coefficients, {n} steps) vs FP64 oracle")
+    _announce(f"LG gradient ({2 * P} coefficients, {n} steps) vs FP64 oracle "
+              f"(STE {'on' if eng.ste else 'off'})")
     theta, hidden = _lg_setup(P, 11)
     g, cfg = oa.g, oa.cfg
     F_t = A.fluence(oa.run(A.lg_field(hidden, g, cfg, P), n), g)
