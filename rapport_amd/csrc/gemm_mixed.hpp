@@ -63,6 +63,17 @@ __host__ __device__ inline int scale_exp(float mx) {
   return se;
 }
 
+// B's power-of-two scale from the max|B| word. The word is read with a vector load from
+// an index the compiler cannot prove uniform: a scalar load would be waited for together
+// with the kernel arguments, before the first operand load is issued, which puts one more
+// memory round trip in front of every tile. Callers issue their first operand loads first.
+FP_DEV void load_scale(const unsigned* __restrict__ bmax_in, int& sb_exp, float& sB) {
+  int z = 0;
+  asm volatile("" : "+v"(z));
+  sb_exp = scale_exp(__uint_as_float(bmax_in[z]));
+  sB = pow2f(sb_exp);
+}
+
 // One K slab of a BM-row (BN-column) tile is BM * 4 h8 fragments.
 template <int BM, int BN, int WM = 2, int WN = 2> struct Smem {
   h8 Ah[2][BM * 4], Al[2][BM * 4];
@@ -108,8 +119,9 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
   const int wn = NARROW ? (wave / WM) * ((TBK + 1) / 2) * 16 : (wave % WN) * (BN / WN);
   const bool wcol0 = wn == 0;
 
-  const int sb_exp = scale_exp(__uint_as_float(*bmax_in));
-  const float sB = pow2f(sb_exp);
+  // B's scale is read after the first operand loads are issued (load_scale below)
+  int sb_exp = 0;
+  float sB = 1.0f;
 
   // staging roles
   const int bc = tid % BN;             // B column within the tile
@@ -264,6 +276,7 @@ FP_DEV void gemm_tile(Smem<BM, BN, WM, WN>& sm, int tm, int tn, const h8* __rest
   // in buffer 0 (visible to all), R0 holds slab s + 1.
   const int nslab = K / BK;
   gload(0, R0);
+  load_scale(bmax_in, sb_exp, sB);
   sstore(0, R0);
   if (nslab > 1) gload(1, R0);
   __syncthreads();
@@ -382,6 +395,228 @@ __global__ __launch_bounds__(512, 1) void hankel_gemm_f16x3_band(
   gemm_tile<256, 128, 4, 2, NPF, TBK>(sm, tm, 0, Yh, Yl, y_scale_exp, B, C, M, K, Ncols, alpha,
                                       rowscale, phase, bmax_in, cmax_out, band,
                                       sl * TBK * 16);
+  if (zero_hi > zero_lo) {
+    const int w4 = (zero_hi - zero_lo) / 4;
+    const long long total = (long long)M * w4;
+    const long long chunk = (total + gridDim.x - 1) / gridDim.x;
+    const long long end = min(total, (long long)(blockIdx.x + 1) * chunk);
+    for (long long i = (long long)blockIdx.x * chunk + threadIdx.x; i < end; i += 512) {
+      const int r = (int)(i / w4), c4 = (int)(i % w4);
+      *reinterpret_cast<float4*>(C + (size_t)r * Ncols + zero_lo + 4 * c4) =
+          make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+
+
+// Band GEMM without a phase (G2 / G4 of the z-step), K split across two wave groups.
+// Tile: 256 rows x TBK 16-column blocks, 8 waves. Waves 0-3 (group 0) multiply the even K
+// slabs, waves 4-7 (group 1) the odd ones; wave w owns rows m0 + 64 (w % 4) of its group's
+// partial sum (all TBK blocks: 4 x TBK accumulators). Each wave loads its own A fragments
+// (Y is stored in fragment order, 1 KB contiguous per fragment) straight from global memory
+// into registers two slabs ahead, so A never passes through LDS and no A fragment is loaded
+// twice per tile; each group splits its slabs' B into LDS (double-buffered per group) and
+// four waves read it. The two partial sums are added through LDS at the end: group 0 stores
+// blocks [0, ceil(TBK / 2)), group 1 the rest. The accumulation order differs from
+// hankel_gemm_f16x3_band (two interleaved partial sums over K), so the results agree with
+// it to FP32 rounding, not bitwise. Needs K % 128 == 0.
+template <int TBK> struct SmemKSplit {
+  static constexpr int RED = 4 * TBK * 4 * 4 * 64;   // partial sums [wr][b][a][v][lane]
+  union {
+    struct {
+      h8 Bh[2][2][128 * 4];   // [group][buffer][fragment]
+      h8 Bl[2][2][128 * 4];
+    } b;
+    float red[RED];
+  } u;
+  float wmax[8];
+};
+
+template <int TBK>
+__global__ __launch_bounds__(512, 1) void hankel_gemm_f16x3_ksplit(
+    const h8* __restrict__ Yh, const h8* __restrict__ Yl, int y_scale_exp,
+    const float* __restrict__ B, float* __restrict__ C, int M, int K, int Ncols, float alpha,
+    const float* __restrict__ rowscale, const unsigned* __restrict__ bmax_in,
+    unsigned* __restrict__ cmax_out, unsigned* __restrict__ reset_slot, int slots, ColBand band,
+    int zero_lo, int zero_hi) {
+  __shared__ SmemKSplit<TBK> sm;
+  constexpr int FA = 4;                        // 16-row blocks per wave
+  constexpr int NB0 = (TBK + 1) / 2;           // blocks stored by group 0
+  if (reset_slot && blockIdx.x == 0 && threadIdx.x == 0) *reset_slot = 0u;
+  const int tilesM = M / 256;
+  const int lin = xcd_remap(blockIdx.x, tilesM * slots);
+  const int tm = lin % tilesM, sl = lin / tilesM;
+  const int m0 = tm * 256, n0 = sl * TBK * 16;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int g = wave >> 2;                     // K group: slabs g, g + 2, g + 4, ...
+  const int wr = wave & 3;                     // wave row
+  const int tg = tid & 255;                    // thread within the group
+  const int lq = lane >> 4, lr = lane & 15;
+
+  // B staging: 2 threads per column, each 2 k groups of 8 rows; columns past the tile's
+  // TBK blocks load its last column (staged, never multiplied)
+  const int bc = tg & 127;
+  const int bkp = tg >> 7;
+  const char* Bc = reinterpret_cast<const char*>(B);
+  const unsigned bcol4 = 4u * (unsigned)band.map(n0 + (bc < TBK * 16 ? bc : TBK * 16 - 1));
+  const unsigned row4 = 4u * (unsigned)Ncols;
+  // A fragment (mb, ks) of this wave: h8 entry (ks (M / 16) + mb) 64 + lane
+  const char* Yhc = reinterpret_cast<const char*>(Yh);
+  const char* Ylc = reinterpret_cast<const char*>(Yl);
+  const unsigned a_slab = 16u * 64u * (unsigned)(M / 16);
+  const unsigned a_base = 16u * (64u * (unsigned)(m0 / 16 + FA * wr) + (unsigned)lane);
+
+  struct AFrag {
+    h8 h[FA], l[FA];
+  };
+  auto gload_a = [&](int ks, AFrag& A) {
+    const unsigned off = (unsigned)ks * a_slab + a_base;
+#pragma unroll
+    for (int a = 0; a < FA; ++a) {
+      A.h[a] = *reinterpret_cast<const h8*>(Yhc + off + 1024u * a);
+      A.l[a] = *reinterpret_cast<const h8*>(Ylc + off + 1024u * a);
+    }
+  };
+  struct BStage {
+    float v[16];
+  };
+  auto gload_b = [&](int ks, BStage& R) {
+    const unsigned k0 = (unsigned)(ks * BK);
+#pragma unroll
+    for (int gg = 0; gg < 2; ++gg) {
+      const unsigned r0 = (k0 + 8u * (unsigned)(2 * bkp + gg)) * row4 + bcol4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        R.v[gg * 8 + j] = *reinterpret_cast<const float*>(Bc + (r0 + (unsigned)j * row4));
+    }
+  };
+  int sb_exp = 0;
+  float sB = 1.0f;
+  auto sstore_b = [&](int buf, const BStage& R) {
+#pragma unroll
+    for (int gg = 0; gg < 2; ++gg) {
+      const int kg = 2 * bkp + gg;
+      h8 hi, lo;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float x = R.v[gg * 8 + j] * sB;
+        const _Float16 h = (_Float16)x;
+        hi[j] = h;
+        lo[j] = (_Float16)(x - (float)h);
+      }
+      const int idx = ((bc >> 4) * 4 + kg) * 16 + (bc & 15);
+      sm.u.b.Bh[g][buf][idx] = hi;
+      sm.u.b.Bl[g][buf][idx] = lo;
+    }
+  };
+
+  f4 acc[FA][TBK];
+#pragma unroll
+  for (int a = 0; a < FA; ++a)
+#pragma unroll
+    for (int b = 0; b < TBK; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // Per column block the FA products Yl Bh, then the FA Yh Bl, then the FA Yh Bh: each
+  // accumulator gets its three updates in the same order as in gemm_tile, with FA - 1
+  // independent MFMAs between two of them. (Measured against one accumulator at a time,
+  // the raised wave priority and the scheduling groups of gemm_tile: 46 vs 50 us at
+  // 1024 x 6240 band columns; those help only when A also comes through LDS.)
+  auto mma = [&](int buf, const AFrag& A) {
+    const h8* bh_s = sm.u.b.Bh[g][buf];
+    const h8* bl_s = sm.u.b.Bl[g][buf];
+#pragma unroll
+    for (int b = 0; b < TBK; ++b) {
+      const int idx = (b * 4 + lq) * 16 + lr;
+      const h8 bh = bh_s[idx];
+      const h8 bl = bl_s[idx];
+#pragma unroll
+      for (int a = 0; a < FA; ++a)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.l[a], bh, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < FA; ++a)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.h[a], bl, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < FA; ++a)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.h[a], bh, acc[a][b], 0, 0, 0);
+    }
+  };
+
+  // Local step t multiplies the group's slab g + 2 t from LDS buffer t & 1 with A set t & 1.
+  // Invariant at the top of an even step: buffer 0 holds step t's B, A0 / A1 steps t and
+  // t + 1, R0 step t + 1's B. T = K / 64 local steps, even.
+  const int T = K / BK / 2;
+  AFrag A0, A1;
+  BStage R0;
+  gload_b(g, R0);
+  gload_a(g, A0);
+  load_scale(bmax_in, sb_exp, sB);
+  sstore_b(0, R0);
+  gload_b(g + 2, R0);
+  gload_a(g + 2, A1);
+  __syncthreads();
+  for (int t = 0; t < T - 2; t += 2) {
+    mma(0, A0);
+    gload_a(g + 2 * (t + 2), A0);
+    sstore_b(1, R0);
+    gload_b(g + 2 * (t + 2), R0);
+    __syncthreads();
+    mma(1, A1);
+    gload_a(g + 2 * (t + 3), A1);
+    sstore_b(0, R0);
+    gload_b(g + 2 * (t + 3), R0);
+    __syncthreads();
+  }
+  mma(0, A0);
+  sstore_b(1, R0);
+  __syncthreads();
+  mma(1, A1);
+
+  // add the two partial sums: each group sends the blocks the other one stores
+  float* red = sm.u.red;
+  auto ridx = [&](int b, int a, int v) { return (((wr * TBK + b) * FA + a) * 4 + v) * 64 + lane; };
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < TBK; ++b) {
+    if ((g == 0) == (b < NB0)) continue;
+#pragma unroll
+    for (int a = 0; a < FA; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[ridx(b, a, v)] = acc[a][b][v];
+  }
+  __syncthreads();
+
+  // epilogue: for v_mfma_f32_16x16x32_f16 lane l, register v holds D[4 (l/16) + v][l % 16]
+  const float unscale = pow2f(-(y_scale_exp + sb_exp));
+  char* Cc = reinterpret_cast<char*>(C);
+  float vmax = 0.0f;
+#pragma unroll
+  for (int a = 0; a < FA; ++a) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + 64 * wr + a * 16 + 4 * lq + v;
+      const float rs = rowscale ? alpha * unscale * rowscale[m] : alpha * unscale;
+      const unsigned rowoff = 4u * (unsigned)m * (unsigned)Ncols;
+#pragma unroll
+      for (int b = 0; b < TBK; ++b) {
+        if ((g == 0) != (b < NB0)) continue;
+        const unsigned n = (unsigned)band.map(n0 + b * 16 + lr);
+        const float val = (acc[a][b][v] + red[ridx(b, a, v)]) * rs;
+        vmax = fmaxf(vmax, fabsf(val));
+        *reinterpret_cast<float*>(Cc + rowoff + 4u * n) = val;
+      }
+    }
+  }
+  if (cmax_out) {
+    for (int d = 32; d >= 1; d >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, d, 64));
+    if (lane == 0) sm.wmax[wave] = vmax;
+    __syncthreads();
+    if (tid == 0) {
+      float mm = sm.wmax[0];
+      for (int w = 1; w < 8; ++w) mm = fmaxf(mm, sm.wmax[w]);
+      atomicMax(cmax_out, __float_as_uint(mm));
+    }
+  }
   if (zero_hi > zero_lo) {
     const int w4 = (zero_hi - zero_lo) / 4;
     const long long total = (long long)M * w4;

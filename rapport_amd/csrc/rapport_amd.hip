@@ -1167,6 +1167,22 @@ int fp_hankel_gemm_mixed_band(const void* Yh, const void* Yl, int y_scale_exp, c
   const auto* yl = reinterpret_cast<const fpgemm16::h8*>(Yl);
   const int zlo = zero_fill ? bp.zlo : 0, zhi = zero_fill ? bp.zhi : 0;
   const dim3 grid((Nr / 256) * bp.slots);
+  if (!phase) {
+    // no phase (G2 / G4): the K-split tile (A fragments from global memory, two partial sums)
+#define FP_KSPLIT(TB)                                                                          \
+  hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3_ksplit<TB>), grid, dim3(512), 0, S(stream),   \
+                     yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols, alpha, rowscale, bmax_in,          \
+                     cmax_out, reset_slot, bp.slots, bp.band, zlo, zhi);                          \
+  break;
+    switch (bp.tbk) {
+      case 4: FP_KSPLIT(4)
+      case 5: FP_KSPLIT(5)
+      case 6: FP_KSPLIT(6)
+      default: FP_KSPLIT(7)
+    }
+#undef FP_KSPLIT
+    return check_launch("hankel_gemm_mixed_band (K split)");
+  }
   constexpr int NPF = FP_MIXED_GEMM_NPF;
 #define FP_BAND(TB)                                                                            \
   hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3_band<TB, NPF>), grid, dim3(512), 0, S(stream), \

@@ -215,3 +215,59 @@ def test_band_gemm_equals_full_gemm_outside_the_mask(case):
             # 16-column alignment: the ends of the masked range may be computed (zeros)
             assert bool(((dead == 7.0) | (dead == 0)).all()) and bool((dead == 7.0).any())
         assert slots[2].item() == slots[1].item()   # same max|C|
+
+
+@pytest.mark.parametrize("case", [C2, dict(C2, N=769), dict(C2, N=513)])
+def test_band_gemm_without_phase_k_split(case):
+    # without a phase (G2 / G4) the banded GEMM runs the K-split tile (two interleaved
+    # partial sums over K, added at the end): the same products as the full GEMM in another
+    # summation order, so the two agree to FP32 accumulation error, not bitwise. Both are
+    # checked against Y @ B in FP64 (4e-6 of max|Y B|, the split-GEMM bound above), with a
+    # row scale, the masked columns zero-filled, and max|C| reduced into its slot.
+    import rapport_amd as R
+    from rapport_amd import _lib
+    from rapport_amd.mixed import split_hankel_fp16, zero_columns
+    _announce(f"K-split band GEMM vs full GEMM and FP64, N={case['N']} Nt={case['Nt']}")
+    cfg = R.Config(**cfg_kwargs(case))
+    g = R.build_grids(cfg)
+    Y = g["Y"]
+    Nr, Nt = Y.shape[0], cfg.Nt
+    zlo, zhi = zero_columns(g["spec_mask"])
+    rng = np.random.default_rng(6)
+    B = (rng.standard_normal((Nr, Nt)) + 1j * rng.standard_normal((Nr, Nt))) * 3e9
+    B[: Nr // 8] *= 1e4
+    B *= g["spec_mask"][None, :]          # a G2 / G4 input: zero on the masked frequencies
+    B64 = B.astype(np.complex64)
+    Bd = _cuda(B64)
+    yh, yl, se = split_hankel_fp16(Y)
+    Yh, Yl = _cuda(yh), _cuda(yl)
+    rs = rng.uniform(0.5, 1.5, Nr)
+    rsd = _cuda(rs.astype(np.float32))
+    slots = torch.zeros(4, dtype=torch.int32, device="cuda")
+    sp = lambda i: ctypes.c_void_p(slots.data_ptr() + 4 * i)  # noqa: E731
+    st = _lib.stream_ptr()
+    _lib.call("fp_absmax_f32", _lib.ptr(Bd), 2 * Nr * Nt, sp(0), st)
+    full = torch.full_like(Bd, complex(7.0, 7.0))
+    band = torch.full_like(Bd, complex(7.0, 7.0))
+    _lib.call("fp_hankel_gemm_mixed", _lib.ptr(Yh), _lib.ptr(Yl), se, _lib.ptr(Bd), _lib.ptr(full),
+              Nr, Nt, ctypes.c_float(0.5), _lib.ptr(rsd), None, sp(0), sp(1), None, st)
+    _lib.call("fp_hankel_gemm_mixed_band", _lib.ptr(Yh), _lib.ptr(Yl), se, _lib.ptr(Bd),
+              _lib.ptr(band), Nr, Nt, ctypes.c_float(0.5), _lib.ptr(rsd), None, sp(0), sp(2),
+              None, zlo, zhi, 1, st)
+    torch.cuda.synchronize()
+    ref = 0.5 * rs.astype(np.float32)[:, None] * (Y @ B64.astype(np.complex128))
+    scale = np.abs(ref).max()
+    got_b = band.cpu().numpy().astype(np.complex128)
+    got_f = full.cpu().numpy().astype(np.complex128)
+    e_b = np.abs(got_b - ref).max() / scale
+    e_f = np.abs(got_f - ref).max() / scale
+    e_bf = np.abs(got_b - got_f).max() / scale
+    _announce(f"  vs FP64: band {e_b:.2e} full {e_f:.2e}; band vs full {e_bf:.2e}")
+    assert e_b < 4e-6 and e_f < 4e-6 and e_bf < 4e-6
+    b = torch.view_as_real(band).reshape(Nr, 2 * Nt)
+    live = torch.ones(2 * Nt, dtype=torch.bool, device="cuda")
+    live[zlo:zhi] = False
+    assert bool((b[:, ~live] == 0).all())            # zero-filled masked columns
+    mb = slots[2:3].cpu().numpy().view(np.float32)[0]
+    mf = slots[1:2].cpu().numpy().view(np.float32)[0]
+    assert abs(mb - mf) <= 4e-6 * mf and mb == float(b.abs().max())
