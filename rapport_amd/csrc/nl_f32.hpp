@@ -18,30 +18,39 @@ using fpfft::LdsSize;
 using fpfft::row_fft;
 typedef cpx<float> c32;
 
-__device__ const float kP10f[18] = {1e0f, 1e1f, 1e2f, 1e3f, 1e4f, 1e5f, 1e6f, 1e7f, 1e8f,
-                                    1e9f, 1e10f, 1e11f, 1e12f, 1e13f, 1e14f, 1e15f, 1e16f, 1e17f};
-__device__ const float kInvP10f[18] = {1e0f,   1e-1f,  1e-2f,  1e-3f,  1e-4f,  1e-5f,
-                                       1e-6f,  1e-7f,  1e-8f,  1e-9f,  1e-10f, 1e-11f,
-                                       1e-12f, 1e-13f, 1e-14f, 1e-15f, 1e-16f, 1e-17f};
+// (10^k, 10^-k) as floats, k = 0 .. 17: the decade boundaries of the knot grid and their
+// reciprocals.
+__device__ const float2 kDec[18] = {
+    {1e0f, 1e0f},     {1e1f, 1e-1f},   {1e2f, 1e-2f},   {1e3f, 1e-3f},   {1e4f, 1e-4f},
+    {1e5f, 1e-5f},    {1e6f, 1e-6f},   {1e7f, 1e-7f},   {1e8f, 1e-8f},   {1e9f, 1e-9f},
+    {1e10f, 1e-10f},  {1e11f, 1e-11f}, {1e12f, 1e-12f}, {1e13f, 1e-13f}, {1e14f, 1e-14f},
+    {1e15f, 1e-15f},  {1e16f, 1e-16f}, {1e17f, 1e-17f}};
 
 FP_DEV float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // PCHIP interval of the (clamped) intensity x on the knot grid 10^k (1 + 0.01 j):
 // returns the coefficient index i, the local coordinate s in [0, 1] and 1 / 10^k.
+// Branch-free: the binary exponent gives the decade k0 = floor(e2 log10 2), which is
+// floor(log10 x) or one below it; both candidate table entries are loaded at once and
+// the comparison with 10^(k0 + 1) picks one. (With a branch per sample the compiler
+// waited for each table load in turn, eight samples one after another.)
 FP_DEV int pchip_piece(float x, int n, float& s, float& inv_pk) {
   const int e2 = (int)((__float_as_uint(x) >> 23) & 0xff) - 127;
   int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;   // floor(e2 log10 2), possibly one low
   if (k > 16) k = 16;
-  if (k < 16 && x >= kP10f[k + 1]) ++k;
-  inv_pk = kInvP10f[k];
+  const float2 d0 = kDec[k], d1 = kDec[k + 1];
+  const bool up = k < 16 && x >= d1.x;
+  k += up ? 1 : 0;
+  inv_pk = up ? d1.y : d0.y;
   const float v = (x * inv_pk - 1.0f) * 100.0f;
   int j = (int)v;
-  if (j < 0) j = 0;
-  if (j > 899) j = 899;
+  j = j < 0 ? 0 : (j > 899 ? 899 : j);
   int i = k * 900 + j;
-  s = v - (float)j;
-  if (i > n - 2) { i = n - 2; s = 1.0f; }
-  s = fminf(fmaxf(s, 0.0f), 1.0f);
+  float sl = v - (float)j;
+  const bool last = i > n - 2;
+  i = last ? n - 2 : i;
+  sl = last ? 1.0f : sl;
+  s = fminf(fmaxf(sl, 0.0f), 1.0f);
   return i;
 }
 

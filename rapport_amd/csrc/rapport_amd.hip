@@ -261,35 +261,47 @@ FP_DEV int group_find(const BatchGroup& g, int item) {
   return lo;
 }
 
-template <int NT, int TPB, bool STE>
-__global__ __launch_bounds__(TPB) void batch_plasma_kernel(const fp_batch_instance* __restrict__ insts,
+// PERS: a persistent grid (blocks loop over the items); otherwise one block per item, with
+// no loop state to carry (the 4096-sample scan then fits the single-instance kernel's
+// register budget: three blocks per CU, no spills).
+template <int NT, int TPB, bool STE, bool PERS>
+__global__ __launch_bounds__(TPB, (PlasmaOcc<float, STE>::value)) void batch_plasma_kernel(const fp_batch_instance* __restrict__ insts,
                                                            BatchGroup g) {
   __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<float, NT, TPB, STE>::bytes];
-  const int total = g.begin[g.n];
+  const int total = PERS ? g.begin[g.n] : (int)blockIdx.x + 1;
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int j = group_find(g, item);
-    const fp_batch_instance& I = insts[g.idx[j]];
+    const fp_batch_instance& I = insts[__builtin_amdgcn_readfirstlane(g.idx[j])];
     const int row = item - g.begin[j];
-    if (row == 0 && threadIdx.x < 3) I.slots[1 + threadIdx.x] = 0u;
-    plasma_row_body<float, NT, TPB, STE, true>(smem, row, I.spec, I.rho,
-                                               STE ? I.rho_s : nullptr, nullptr, I.pp,
-                                               I.W_lut, I.Ws_lut, I.tw, I.status);
-    __syncthreads();
+    // every descriptor field is read before the first store (the slot clear): reads that
+    // follow a store through a pointer could alias the descriptors, so the compiler would
+    // make them vector loads, holding each pointer in VGPRs (74 spilled VGPRs at 4096)
+    const fp_plasma_params pp = I.pp;
+    const float* spec = I.spec;
+    double *rho = I.rho, *rho_s = STE ? I.rho_s : nullptr;
+    const double *W = I.W_lut, *Ws = I.Ws_lut;
+    const float* tw = I.tw;
+    int* status = I.status;
+    unsigned* slots = I.slots;
+    if (row == 0 && threadIdx.x < 3) slots[1 + threadIdx.x] = 0u;
+    plasma_row_body<float, NT, TPB, STE, true>(smem, row, spec, rho, rho_s, nullptr, pp, W, Ws,
+                                               tw, status);
+    if (PERS) __syncthreads();
   }
 }
 
-template <int NT, int TPB, bool RS>
+template <int NT, int TPB, bool RS, bool PERS>
 __global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void batch_nl_kernel(
     const fp_batch_instance* __restrict__ insts, BatchGroup g) {
   __shared__ fpnl32::StepSmem<NT> sm;
-  const int total = g.begin[g.n];
+  const int total = PERS ? g.begin[g.n] : (int)blockIdx.x + 1;
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int j = group_find(g, item);
     const fp_batch_instance& I = insts[g.idx[j]];
     const int row = item - g.begin[j];
     fpnl32::nl_step_row<NT, TPB, RS>(sm, row, I.w0, I.w0, I.rho, RS ? I.rho_s : nullptr, 1, 1,
                                      I.np, I.nt, I.slots + 2);
-    __syncthreads();
+    if (PERS) __syncthreads();
   }
 }
 
@@ -607,13 +619,21 @@ inline dim3 grid_of(const BatchGroup& g, int persistent) {
   return dim3(persistent > 0 && persistent < total ? persistent : total);
 }
 
-int launch_batch_plasma(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
+inline bool persistent(const BatchGroup& g, int blocks) { return blocks > 0 && blocks < g.begin[g.n]; }
+
+template <int NT, bool STE>
+void launch_batch_plasma_t(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
   const dim3 grid = grid_of(L.g, P.grid[0]);
+  if (persistent(L.g, P.grid[0]))
+    hipLaunchKernelGGL((batch_plasma_kernel<NT, 256, STE, true>), grid, dim3(256), 0, st, P.dev, L.g);
+  else
+    hipLaunchKernelGGL((batch_plasma_kernel<NT, 256, STE, false>), grid, dim3(256), 0, st, P.dev, L.g);
+}
+
+int launch_batch_plasma(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
 #define FP_BP(NT)                                                                               \
-  if (L.ste)                                                                                    \
-    hipLaunchKernelGGL((batch_plasma_kernel<NT, 256, true>), grid, dim3(256), 0, st, P.dev, L.g); \
-  else                                                                                          \
-    hipLaunchKernelGGL((batch_plasma_kernel<NT, 256, false>), grid, dim3(256), 0, st, P.dev, L.g); \
+  if (L.ste) launch_batch_plasma_t<NT, true>(P, L, st);                                         \
+  else launch_batch_plasma_t<NT, false>(P, L, st);                                              \
   break;
   switch (L.Nt) {
     case 512: FP_BP(512)
@@ -629,10 +649,14 @@ int launch_batch_plasma(const fp_batch_plan& P, const fp_batch_plan::Launch& L, 
 template <int NT, int TPB>
 void launch_batch_nl_t(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
   const dim3 grid = grid_of(L.g, P.grid[2]);
-  if (L.ste)
-    hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, true>), grid, dim3(TPB), 0, st, P.dev, L.g);
-  else
-    hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, false>), grid, dim3(TPB), 0, st, P.dev, L.g);
+  const bool pers = persistent(L.g, P.grid[2]);
+  if (L.ste) {
+    if (pers) hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, true, true>), grid, dim3(TPB), 0, st, P.dev, L.g);
+    else hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, true, false>), grid, dim3(TPB), 0, st, P.dev, L.g);
+  } else {
+    if (pers) hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, false, true>), grid, dim3(TPB), 0, st, P.dev, L.g);
+    else hipLaunchKernelGGL((batch_nl_kernel<NT, TPB, false, false>), grid, dim3(TPB), 0, st, P.dev, L.g);
+  }
 }
 
 int launch_batch_nl(const fp_batch_plan& P, const fp_batch_plan::Launch& L, hipStream_t st) {
