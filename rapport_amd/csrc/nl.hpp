@@ -24,26 +24,31 @@ __device__ const double kPow10[18] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e
 __device__ const double kInvPow10[18] = {1e0, 1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8,
                                          1e-9, 1e-10, 1e-11, 1e-12, 1e-13, 1e-14, 1e-15, 1e-16,
                                          1e-17};
-FP_DEV double p10(int k) { return kPow10[k]; }
 
 // Interval i with knots[i] <= x < knots[i+1] (x in [knots[0], knots[n-1]];
-// x == knots[n-1] maps to n-2, as scipy.interpolate.PPoly does).
-FP_DEV int pchip_interval(double x, const double* __restrict__ knots, int n) {
-  // decade from the binary exponent: floor(e2 log10 2) = (e2 * 1233) >> 12 for
-  // 0 <= e2 < 1024, then an exact fix-up against the powers of ten
+// x == knots[n-1] maps to n-2, as scipy.interpolate.PPoly does), and xi = knots[i].
+// Branch-free, so the table loads of a thread's samples overlap: the decade follows from
+// the binary exponent (floor(e2 log10 2) = (e2 * 1233) >> 12, floor(log10 x) or one below
+// it, settled by one comparison with 10^(k+1)); the index from the scaled mantissa is then
+// at most one interval off, and one comparison each way with the knots makes it exact
+// (checked on the CPU against searchsorted on every knot, its two neighbouring doubles and
+// 2e6 random intensities). The loop form waited for each load in turn.
+FP_DEV int pchip_interval(double x, const double* __restrict__ knots, int n, double& xi) {
   const int e2 = (int)((__double_as_longlong(x) >> 52) & 0x7ff) - 1023;
   int k = e2 <= 0 ? 0 : (e2 * 1233) >> 12;
   if (k > 16) k = 16;
-  while (k < 16 && x >= p10(k + 1)) ++k;
-  while (k > 0 && x < p10(k)) --k;
-  int j = (int)((x * kInvPow10[k] - 1.0) * 100.0);
-  if (j < 0) j = 0;
-  if (j > 899) j = 899;
+  const double p1 = kPow10[k + 1], ip0 = kInvPow10[k], ip1 = kInvPow10[k + 1];
+  const bool up = k < 16 && x >= p1;
+  k += up ? 1 : 0;
+  int j = (int)((x * (up ? ip1 : ip0) - 1.0) * 100.0);
+  j = j < 0 ? 0 : (j > 899 ? 899 : j);
   int i = k * 900 + j;
-  if (i > n - 2) i = n - 2;
-  while (i < n - 2 && knots[i + 1] <= x) ++i;
-  while (i > 0 && knots[i] > x) --i;
-  return i;
+  i = i > n - 2 ? n - 2 : i;
+  const double km = knots[i > 0 ? i - 1 : 0], k0 = knots[i], k1 = knots[i + 1];
+  const bool inc = i < n - 2 && k1 <= x;
+  const bool dec = i > 0 && k0 > x;
+  xi = inc ? k1 : (dec ? km : k0);
+  return inc ? i + 1 : (dec ? i - 1 : i);
 }
 
 // |PCHIP(I)| with nan_to_num, FP64 power basis in dx = x - x_i.
@@ -52,8 +57,9 @@ FP_DEV double keldysh_W(double absu2, const fp_nl_params& p, const fp_nl_tables&
   x = fmin(fmax(x, p.Imin), p.Imax);   // cp.clip
   if (!fp_isfinite(x)) x = p.Imax;     // clip of +inf; NaN handled below
   const int n = p.n_knots;
-  const int i = pchip_interval(x, t.knots, n);
-  const double dx = x - t.knots[i];
+  double xi;
+  const int i = pchip_interval(x, t.knots, n, xi);
+  const double dx = x - xi;
   const int m = n - 1;
   const double c0 = t.pcoef[i], c1 = t.pcoef[m + i], c2 = t.pcoef[2 * m + i], c3 = t.pcoef[3 * m + i];
   const double w = fabs(((c0 * dx + c1) * dx + c2) * dx + c3);
@@ -81,12 +87,13 @@ FP_DEV void photo_alpha(double absu2, double rho, const fp_nl_params& p, const f
 // given, al = alpha(us). Register state is kept to us and two transform
 // buffers: rho (and rho_s) are re-read from the row in global memory (L2
 // resident), |us|^2 is recomputed where needed, the Kerr intensity is formed in
-// the transform buffer, and alpha is re-evaluated for the final alpha*u term.
+// the transform buffer, and the photo-ionisation rate of each sample is parked in the
+// LDS row sph (NT doubles) for the final alpha*u term, which is alpha of the same field.
 template <int NT, int TPB, bool RS>
 FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double* __restrict__ rho_row,
                       const double* __restrict__ rhos_row, cpx<double> (&k)[NT / TPB],
                       double* al, const fp_nl_params& p, const fp_nl_tables& t,
-                      cpx<double>* lds) {
+                      cpx<double>* lds, double* sph) {
   constexpr int E = NT / TPB;
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));   // recompute table addresses here rather than hoist them
@@ -102,6 +109,7 @@ FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double* __restric
       const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
       double ph, a;
       photo_alpha(a2, rho_row[tid + s * TPB], p, t, ph, a);
+      sph[tid + s * TPB] = ph;
       buf[s] = ph * us[s];
     }
     row_fft<double, NT, TPB, false>(buf, lds, tw);
@@ -149,8 +157,11 @@ FP_DEV void split_row(const cpx<double> (&us)[NT / TPB], const double* __restric
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const double a2 = us[s].x * us[s].x + us[s].y * us[s].y;
-    double ph, a;
-    photo_alpha(a2, rho_row[tid + s * TPB], p, t, ph, a);
+    // alpha as photo_alpha forms it, from the rate computed above for this field
+    double a = p.en_pi_loss ? sph[tid + s * TPB] : 0.0;
+    if (p.en_plasma_absorb) a = a + p.plasma_pref * rho_row[tid + s * TPB];
+    a = nan0(a);
+    a = a > 0.0 ? a : 0.0;
     if (al) al[s] = a;
     cpx<double> r = mk<double>(k[s].x * invN + a * us[s].x, k[s].y * invN + a * us[s].y);
     if (!(a2 >= 1e-30)) r = mk<double>(0.0, 0.0);   // where(absu2 < 1e-30, 0, rhs)
@@ -180,12 +191,14 @@ __global__ __launch_bounds__(TPB) void split_kernel(const double* __restrict__ u
                                                      double* __restrict__ alpha, fp_nl_params p,
                                                      fp_nl_tables t) {
   __shared__ cpx<double> lds[LdsSize<NT>::value];
+  __shared__ double sph[NT];
   constexpr int E = NT / TPB;
   const size_t row = blockIdx.x;
   cpx<double> us[E], k[E];
   double al[E];
   load_row<NT, TPB>(u, row, us);
-  split_row<NT, TPB, RS>(us, rho + row * NT, RS ? rhos + row * NT : nullptr, k, al, p, t, lds);
+  split_row<NT, TPB, RS>(us, rho + row * NT, RS ? rhos + row * NT : nullptr, k, al, p, t, lds,
+                         sph);
   store_row<NT, TPB>(rhs, row, k);
   if (alpha) {
 #pragma unroll
@@ -217,6 +230,7 @@ __global__ __launch_bounds__(TPB) void nl_step_kernel(const double* in, double* 
                                                        fp_nl_params p, fp_nl_tables t) {
   __shared__ cpx<double> lds[LdsSize<NT>::value];
   __shared__ cpx<double> ub[NT];
+  __shared__ double sph[NT];   // 160 KB in all at NT = 4096: one block per CU, as before
   constexpr int E = NT / TPB;
   const int tid = threadIdx.x;
   const size_t row = blockIdx.x;
@@ -241,7 +255,7 @@ __global__ __launch_bounds__(TPB) void nl_step_kernel(const double* in, double* 
   // instruction stream: the stage loop is not unrolled.
 #pragma unroll 1
   for (int stg = 0; stg < 4; ++stg) {
-    split_row<NT, TPB, RS>(us, rr, rs, k, nullptr, p, t, lds);
+    split_row<NT, TPB, RS>(us, rr, rs, k, nullptr, p, t, lds, sph);
     const double w = (stg == 0 || stg == 3) ? 1.0 : 2.0;
     const double cdz = (stg < 2) ? 0.5 * dz : dz;
 #pragma unroll
