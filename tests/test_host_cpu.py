@@ -318,3 +318,36 @@ def test_exact_twiddle_table_layout(tmp_path):
         assert np.abs(tab[..., 1] + tab[..., 3] - z.imag).max() < 1e-14
         assert np.abs(tab[..., 0] - z.real).max() <= 2 ** -24
     assert len(offs) > 0
+
+
+def test_pchip_interval_estimate_is_one_fixup_from_searchsorted():
+    # the FP64 nonlinear step finds the PCHIP interval without loops: decade from the binary
+    # exponent ((e2 * 1233) >> 12, one step up against 10^(k+1)), index from the scaled
+    # mantissa, then one comparison each way with the knots. That is exact only if the
+    # index before the fix-up is at most one interval off; checked here on the product's
+    # knots, every knot and its two neighbouring doubles, and random intensities, with the
+    # product's (x * 10^-k - 1) * 100 rounded once (fused) or twice.
+    from rapport_amd.grids import intensity_knots
+    knots = intensity_knots()
+    n = len(knots)
+    P10 = 10.0 ** np.arange(18)
+    IP10 = 10.0 ** -np.arange(18.0)
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([knots, np.nextafter(knots, 0), np.nextafter(knots, np.inf),
+                         10 ** rng.uniform(0, 17, 200_000)])
+    xs = xs[(xs >= knots[0]) & (xs <= knots[-1])]
+    e2 = ((xs.view(np.int64) >> 52) & 0x7ff) - 1023
+    k = np.minimum(np.where(e2 <= 0, 0, (e2 * 1233) >> 12), 16)
+    up = (k < 16) & (xs >= P10[k + 1])
+    k = k + up
+    true = np.clip(np.searchsorted(knots, xs, side="right") - 1, 0, n - 2)
+    for fused in (False, True):
+        if fused:
+            v = (np.longdouble(xs) * np.longdouble(IP10[k]) - 1).astype(np.float64) * 100.0
+        else:
+            v = (xs * IP10[k] - 1.0) * 100.0
+        i = np.minimum(k * 900 + np.clip(v.astype(np.int64), 0, 899), n - 2)
+        assert np.abs(i - true).max() <= 1
+        inc = (i < n - 2) & (knots[np.minimum(i + 1, n - 1)] <= xs)
+        dec = (i > 0) & (knots[i] > xs)
+        assert np.array_equal(np.where(inc, i + 1, np.where(dec, i - 1, i)), true)
