@@ -494,8 +494,16 @@ def _roofline(prec, r):
     flops = 2.0 * Nr * Nr * cols
     ach = flops / (r["gemm_ms"] * 1e-3) / 1e12
     peak = PEAK_TFLOPS[prec]
-    kname = "hankel_gemm_f64_band" if prec == "fp64" else "hankel_gemm_f16x3_band"
-    tr = _pmc_traffic(kname)
+    # the mixed step's G1 / G3 (phase epilogue) run the band tile, G2 / G4 the K-split tile:
+    # gemm_ms is the average over the four launches, and so is the traffic
+    knames = (["hankel_gemm_f64_band"] if prec == "fp64" else
+              ["hankel_gemm_f16x3_band", "hankel_gemm_f16x3_ksplit"])
+    trs = [_pmc_traffic(k) for k in knames]
+    tr = None
+    if all(trs):
+        tr = {key: sum(t[key] for t in trs) / len(trs) for key in ("bytes", "read_bytes", "write_bytes")}
+        tr.update(kernels=[t["kernel"] for t in trs], source=trs[0]["source"])
+    kname = " + ".join(knames)
     roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
                 traffic=tr["bytes"] if tr else None, kernel=kname,
                 flops_per_launch=flops, computed_real_columns=cols,
