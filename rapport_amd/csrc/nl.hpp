@@ -210,13 +210,31 @@ __global__ __launch_bounds__(TPB) void split_kernel(const double* __restrict__ u
 template <int E, int TPB>
 FP_DEV void alpha_half(cpx<double> (&u)[E], const double* __restrict__ rho_row,
                        const fp_nl_params& p, const fp_nl_tables& t) {
+  // photo_alpha for the E samples, with its uniform flags tested once outside the
+  // per-sample loops (per-sample branches serialised the samples' table loads)
+  double a[E];
+  if (p.en_pi_loss) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const double a2 = u[s].x * u[s].x + u[s].y * u[s].y;
+      const double W = keldysh_W(a2, p, t);
+      double depl = 1.0 - rho_row[threadIdx.x + s * TPB] / p.rho_max;
+      depl = fmin(fmax(depl, 0.0), 1.0);
+      a[s] = W * p.photo_pref / (a2 + 1e-30) * depl;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) a[s] = 0.0;
+  }
+  if (p.en_plasma_absorb) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) a[s] = a[s] + p.plasma_pref * rho_row[threadIdx.x + s * TPB];
+  }
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const double a2 = u[s].x * u[s].x + u[s].y * u[s].y;
-    double ph, a;
-    photo_alpha(a2, rho_row[threadIdx.x + s * TPB], p, t, ph, a);
-    const double f = exp(-0.5 * p.dz * a);
-    u[s] = f * u[s];
+    double as = nan0(a[s]);
+    as = as > 0.0 ? as : 0.0;
+    u[s] = exp(-0.5 * p.dz * as) * u[s];
   }
 }
 

@@ -77,15 +77,6 @@ FP_DEV float photo_rate(float a2, float rho, const fp_nl_params& p, const fp_nl_
   return nan0(w) * depl * rcp(a2 + 1e-30f);
 }
 
-FP_DEV float alpha_of(float a2, float rho, const fp_nl_params& p, const fp_nl_tables_f32& t,
-                      float& photo) {
-  photo = p.en_pi_loss ? photo_rate(a2, rho, p, t) : 0.0f;
-  float a = photo;
-  if (p.en_plasma_absorb) a = a + (float)p.plasma_pref * rho;
-  a = nan0(a);
-  return a > 0.0f ? a : 0.0f;
-}
-
 // rho_row: the row's electron density as float (LDS in the fused step);
 // rhos_row: the STE density row in global memory (only read when RS).
 template <int NT, int TPB, bool RS>
@@ -207,12 +198,29 @@ template <int E, int TPB>
 FP_DEV void alpha_half(c32 (&u)[E], const float* rho_row, const fp_nl_params& p,
                        const fp_nl_tables_f32& t) {
   const float hdz = (float)(-0.5 * p.dz);
+  const int tid = threadIdx.x & (TPB - 1);
+  // alpha of split() (photo-ionisation rate + plasma absorption, NaN -> 0, clipped at 0) for
+  // the E samples, with its uniform flags tested once outside the per-sample loops:
+  // per-sample branches kept each sample's two table loads in a region of its own, waited
+  // for one sample after another
+  float a[E];
+  if (p.en_pi_loss) {
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+      a[s] = photo_rate(u[s].x * u[s].x + u[s].y * u[s].y, rho_row[tid + s * TPB], p, t);
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) a[s] = 0.0f;
+  }
+  if (p.en_plasma_absorb) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) a[s] = a[s] + (float)p.plasma_pref * rho_row[tid + s * TPB];
+  }
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const float a2 = u[s].x * u[s].x + u[s].y * u[s].y;
-    float ph;
-    const float a = alpha_of(a2, rho_row[(threadIdx.x & (TPB - 1)) + s * TPB], p, t, ph);
-    u[s] = expf(hdz * a) * u[s];
+    float as = nan0(a[s]);
+    as = as > 0.0f ? as : 0.0f;
+    u[s] = expf(hdz * as) * u[s];
   }
 }
 
