@@ -127,8 +127,29 @@ constexpr int exact_table_offset() {
   return off;
 }
 
-template <typename T, int N, int TPB, bool INV, int Ns, bool EXACT = false>
-FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw, int tid) {
+// FP64, one butterfly per thread and pass (E = R): the table twiddle w^1 of every pass is
+// loaded when the transform starts (pre[d] for the pass of depth d), not at the top of its
+// pass, so the loads overlap each other and the first passes instead of stalling each pass
+// (the FP64 row kernels run at two waves per SIMD).
+template <typename T, int N, int TPB>
+struct TwPre {
+  static constexpr int kMaxPasses = 16;   // log2 N passes at most (radix 2, N <= 65536)
+  cpx<T> w[kMaxPasses];
+};
+
+template <typename T, int N, int TPB, int Ns, int D>
+FP_DEV void prefetch_twiddles(TwPre<T, N, TPB>& pre, const cpx<T>* __restrict__ tw, int tid) {
+  constexpr int E = N / TPB;
+  constexpr int REM = N / Ns;
+  constexpr int R = REM >= E ? E : REM;
+  static_assert(D < TwPre<T, N, TPB>::kMaxPasses, "more transform passes than prefetch slots");
+  if constexpr (Ns > 1 && E / R == 1) pre.w[D] = tw[(tid & (Ns - 1)) * (N / (Ns * R))];
+  if constexpr (Ns * R < N) prefetch_twiddles<T, N, TPB, Ns * R, D + 1>(pre, tw, tid);
+}
+
+template <typename T, int N, int TPB, bool INV, int Ns, bool EXACT = false, int D = 0>
+FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw, int tid,
+                   const TwPre<T, N, TPB>* pre = nullptr) {
   constexpr int E = N / TPB;
   constexpr int REM = N / Ns;
   constexpr int R = REM >= E ? E : REM;
@@ -162,7 +183,7 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
         }
       } else if (sizeof(T) == 8) {
         // FP64: powers of one table twiddle (rounding growth ~R ulp is below FP64 needs)
-        cpx<T> w1 = tw[k * STRIDE];
+        cpx<T> w1 = (Q == 1 && pre) ? pre->w[D] : tw[k * STRIDE];
         if (INV) w1 = conjx(w1);
         cpx<T> w = w1;
 #pragma unroll
@@ -222,7 +243,7 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
       for (int s = 0; s < E; ++s) v[s] = lds[lds_pad(tid + s * TPB)];
     }
     __syncthreads();
-    passes<T, N, TPB, INV, Ns * R, EXACT>(v, lds, tw, tid);
+    passes<T, N, TPB, INV, Ns * R, EXACT, D + 1>(v, lds, tw, tid, pre);
   }
 }
 
@@ -237,7 +258,13 @@ FP_DEV void row_fft(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict_
   // them in registers across the whole kernel.
   int tid = (int)threadIdx.x & (TPB - 1);
   asm volatile("" : "+v"(tid));
-  passes<T, N, TPB, INV, 1, EXACT>(v, lds, tw, tid);
+  if constexpr (sizeof(T) == 8 && !EXACT) {
+    TwPre<T, N, TPB> pre;
+    prefetch_twiddles<T, N, TPB, 1, 0>(pre, tw, tid);
+    passes<T, N, TPB, INV, 1, EXACT>(v, lds, tw, tid, &pre);
+  } else {
+    passes<T, N, TPB, INV, 1, EXACT>(v, lds, tw, tid);
+  }
 }
 
 }  // namespace fpfft
