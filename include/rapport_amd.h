@@ -275,10 +275,12 @@ int fp_hankel_gemm_band(const double* Yt, const double* B, double* C, int Nr, in
  * spectral mask is 0 there, so the G1 / G3 outputs (phase x mask) and the G2 / G4 inputs
  * vanish. The remaining columns are tiled so that the tiles fill the GPU once; columns in
  * the range are not written, or, with zero_fill, set to 0 (needed where a later row
- * kernel reads the full row). Same results as fp_hankel_gemm_mixed in the columns
- * computed. When the range is too narrow to gain anything (or Nr % 256 != 0) it is
- * fp_hankel_gemm_mixed. Replaces cp.dot(self.Y, u) (D3/sim/operators.py:48,53) in the
- * z-step pipeline. */
+ * kernel reads the full row). With a phase, the same results as fp_hankel_gemm_mixed in the
+ * columns computed; without one (G2 / G4 of the z-step) the tile splits K between two wave
+ * groups, so the results agree with fp_hankel_gemm_mixed to FP32 accumulation rounding (the
+ * products are the same, the summation order differs). When the range is too narrow to gain
+ * anything (or Nr % 256 != 0) it is fp_hankel_gemm_mixed. Replaces cp.dot(self.Y, u)
+ * (D3/sim/operators.py:48,53) in the z-step pipeline. */
 int fp_hankel_gemm_mixed_band(const void* Yh, const void* Yl, int y_scale_exp, const float* B,
                               float* C, int Nr, int Nt, float alpha, const float* rowscale,
                               const float* phase, const unsigned* bmax_in, unsigned* cmax_out,
