@@ -378,13 +378,15 @@ enum { FP_PASS_PLASMA = 0, FP_PASS_G1 = 1, FP_PASS_G2 = 2, FP_PASS_NL = 3, FP_PA
 
 /* Validates the n descriptors (host array), copies them to the device and groups
  * the instances per pass. Default launch configuration: GEMM tile 128 x 128 (64
- * rows when Nr % 128 != 0), nonlinear kernel Nt/8 threads, one block per work item. */
+ * rows when Nr % 128 != 0), nonlinear kernel Nt/8 threads (Nt/16 at Nt = 4096), one block
+ * per work item. */
 int fp_batch_plan_create(const fp_batch_instance* inst, int n, fp_batch_plan** plan);
 int fp_batch_plan_destroy(fp_batch_plan* plan);
 
 /* Launch configuration of instance i: GEMM tile bm x bn (64/128 x 64/128 with 4 waves,
  * 128 x 256 with 4 waves, or 256 x 128 with 8 waves; Nr % bm == 0, 2 Nt % bn == 0) and
- * nonlinear threads per row nl_tpb (Nt/8 or Nt/4); 0 keeps the current value. */
+ * nonlinear threads per row nl_tpb (Nt/8 or Nt/4, or Nt/16 at Nt = 4096); 0 keeps the
+ * current value. */
 int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb);
 
 /* Grid of the launches of pass kind 0 = plasma, 1 = GEMM, 2 = nonlinear: 0 = one

@@ -100,7 +100,7 @@ def tune(batch, warmup=2, reps=5, grids=True, verbose=False):
         plan = _sub_plan(batch, idx)
         try:
             best = None
-            for tpb in (Nt // 8, Nt // 4):
+            for tpb in ((Nt // 16,) if Nt == 4096 else ()) + (Nt // 8, Nt // 4):
                 for k in range(len(idx)):
                     call("fp_batch_set_config", plan, k, 0, 0, tpb)
                 t = _time(lambda: call("fp_batch_pass", plan, 3, st), warmup, reps)

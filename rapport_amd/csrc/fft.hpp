@@ -13,8 +13,8 @@
 // inverse has +i and is NOT scaled here (callers fold the 1/N).
 //
 // Twiddles come from a per-N table tw[m] = exp(-2 pi i m / N), m < N. The exact float
-// variant (EXACT, E = 4 or 8 samples per thread) reads compact per-pass tables stored after
-// tw[2 N] instead (exact_table_offset).
+// variant (EXACT, E = 4, 8 or 16 samples per thread) reads compact per-pass tables stored
+// after tw[2 N] instead (exact_table_offset).
 #pragma once
 #include "common.hpp"
 
@@ -108,9 +108,10 @@ template <typename T, int R, bool INV, bool EXACT = false> FP_DEV void dft(cpx<T
 
 // Compact twiddle tables of the exact float transform, stored after the 2 N entries of
 // the twiddle table: for a row held E samples per thread the passes are radix
-// R = min(E, N / Ns) at Ns = 1, R, R R', ...; each twiddled pass (Ns > 1) stores, for k < Ns
-// and r = 1 .. R-1, the float4 (w^r rounded, residual), w = exp(-2 pi i k N / (Ns R) / N).
-// The table for E = 8 comes first, then the one for E = 4.
+// R = min(E, N / Ns) at Ns = 1, R, R R', ...; each twiddled pass (Ns > 1) stores, for
+// r = 1 .. R-1 and k < Ns (entry (r - 1) Ns + k), the float4 (w^r rounded, residual),
+// w = exp(-2 pi i k N / (Ns R) / N).
+// The table for E = 8 comes first, then the one for E = 4, then the one for E = 16.
 constexpr int exact_pass_entries(int N, int E, int Ns) {
   return Ns >= N ? 0
                  : (Ns > 1 ? Ns * ((N / Ns >= E ? E : N / Ns) - 1) : 0) +
@@ -118,7 +119,8 @@ constexpr int exact_pass_entries(int N, int E, int Ns) {
 }
 template <int N, int E, int NsTarget>
 constexpr int exact_table_offset() {
-  int off = E == 4 ? exact_pass_entries(N, 8, 1) : 0;
+  int off = E == 8 ? 0 : exact_pass_entries(N, 8, 1);
+  if (E == 16) off += exact_pass_entries(N, 4, 1);
   for (int ns = 1; ns < NsTarget;) {
     const int r = N / ns >= E ? E : N / ns;
     if (ns > 1) off += ns * (r - 1);
@@ -166,14 +168,15 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
     if (Ns > 1) {
       if (EXACT && sizeof(T) == 4) {
         // every power from the pass's compact table (exact_table_offset): one float4
-        // (rounded w^r, residual) per power, the powers of a butterfly contiguous, so the
-        // loads of a wave are contiguous; products fused with the residual: no rounding
-        // of the twiddles is raised to a power, and the transform has no coherent gain
+        // (rounded w^r, residual) per power, stored power-major ([r - 1][k]) so the lanes
+        // of a wave, whose k are consecutive (or repeat), read one contiguous run per
+        // power; products fused with the residual: no rounding of the twiddles is raised
+        // to a power, and the transform has no coherent gain
         const float4* pt = reinterpret_cast<const float4*>(tw + 2 * N) +
-                           exact_table_offset<N, E, Ns>() + k * (R - 1);
+                           exact_table_offset<N, E, Ns>() + k;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const float4 e = pt[r - 1];
+          const float4 e = pt[(r - 1) * Ns];
           const T hy = INV ? -e.y : e.y, ly = INV ? -e.w : e.w;
           const cpx<T> x = a[r];
           const T re_lo = __builtin_fmaf(x.x, e.z, -x.y * ly);
@@ -252,7 +255,8 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
 template <typename T, int N, int TPB, bool INV, bool EXACT = false>
 FP_DEV void row_fft(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__ tw) {
   static_assert(N % TPB == 0, "N must be a multiple of TPB");
-  static_assert(!EXACT || N / TPB == 4 || N / TPB == 8, "exact transform: 4 or 8 samples per thread");
+  static_assert(!EXACT || N / TPB == 4 || N / TPB == 8 || N / TPB == 16,
+                "exact transform: 4, 8 or 16 samples per thread");
   // Opaque copy of the thread index: keeps the compiler from hoisting this
   // transform's LDS / twiddle addresses out of enclosing loops and holding
   // them in registers across the whole kernel.

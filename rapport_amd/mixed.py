@@ -64,14 +64,14 @@ def split_hankel_fp16(Y):
 def _exact_pass_table(N, E):
     """Compact twiddles of the exact float transform for rows of E samples per thread
     (fft.hpp, exact_table_offset): per twiddled pass (radix R = min(E, N / Ns) at
-    Ns = 1, R, ...; Ns > 1), for k < Ns and r = 1 .. R-1, the pair (w^r rounded to
-    complex64, residual), w = exp(-2 pi i k / (Ns R)), as 4 floats."""
+    Ns = 1, R, ...; Ns > 1), power-major (r = 1 .. R-1, then k < Ns), the pair (w^r
+    rounded to complex64, residual), w = exp(-2 pi i k / (Ns R)), as 4 floats."""
     out, ns = [], 1
     while ns < N:
         R = min(E, N // ns)
         if ns > 1:
-            k = np.arange(ns)[:, None]
-            r = np.arange(1, R)[None, :]
+            k = np.arange(ns)[None, :]
+            r = np.arange(1, R)[:, None]
             z = np.exp(-2j * np.pi * k * r / (ns * R))
             hi = z.astype(np.complex64)
             lo = (z - hi.astype(np.complex128)).astype(np.complex64)
@@ -83,12 +83,13 @@ def _exact_pass_table(N, E):
 def exact_twiddles(N):
     """Twiddle table of the complex64 kernels: tw[m] = exp(-2 pi i m / N) rounded (m < N),
     their residuals tw[N + m] = exp(-2 pi i m / N) - tw[m], then the compact per-pass tables
-    of the exact transform for 8 and for 4 samples per thread (complex64 array)."""
+    of the exact transform for 8, for 4 and for 16 samples per thread (complex64 array)."""
     z = np.exp(-2j * np.pi * np.arange(N) / N)
     hi = z.astype(np.complex64)
     lo = (z - hi.astype(np.complex128)).astype(np.complex64)
-    t8, t4 = _exact_pass_table(N, 8), _exact_pass_table(N, 4)
-    flat = np.concatenate([hi.view(np.float32), lo.view(np.float32), t8.ravel(), t4.ravel()])
+    t8, t4, t16 = (_exact_pass_table(N, e) for e in (8, 4, 16))
+    flat = np.concatenate([hi.view(np.float32), lo.view(np.float32), t8.ravel(), t4.ravel(),
+                           t16.ravel()])
     return np.ascontiguousarray(flat.view(np.complex64))
 
 

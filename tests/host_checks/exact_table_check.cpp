@@ -16,9 +16,11 @@ void pass_offsets() {
 
 template <int N>
 void one() {
-  std::printf("t %d %d %d\n", N, fpfft::exact_pass_entries(N, 8, 1), fpfft::exact_pass_entries(N, 4, 1));
+  std::printf("t %d %d %d %d\n", N, fpfft::exact_pass_entries(N, 8, 1), fpfft::exact_pass_entries(N, 4, 1),
+              fpfft::exact_pass_entries(N, 16, 1));
   pass_offsets<N, 8, 1>();
   pass_offsets<N, 4, 1>();
+  pass_offsets<N, 16, 1>();
 }
 
 int main() {

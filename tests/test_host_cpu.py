@@ -283,7 +283,7 @@ def test_nonlinear_entry_points_reject_nonpositive_rows(name):
 
 
 def test_exact_twiddle_table_layout(tmp_path):
-    """The compact twiddle tables that mixed.exact_twiddles appends (per pass, k, r: w^r
+    """The compact twiddle tables that mixed.exact_twiddles appends (per pass, r, k: w^r
     rounded and its residual) sit where the exact transform reads them: the table sizes and
     every pass offset of fft.hpp (compiled host-only) match the host layout, and each entry
     is exp(-2 pi i k r / (Ns R)) split into a complex64 part and its residual."""
@@ -301,18 +301,19 @@ def test_exact_twiddle_table_layout(tmp_path):
     for ln in out.splitlines():
         f = ln.split()
         if f[0] == "t":
-            sizes[int(f[1])] = (int(f[2]), int(f[3]))
+            sizes[int(f[1])] = (int(f[2]), int(f[3]), int(f[4]))
         else:
             offs.append(tuple(int(x) for x in f[1:]))
-    for N, (n8, n4) in sizes.items():
+    for N, (n8, n4, n16) in sizes.items():
         assert n8 == len(_exact_pass_table(N, 8)) and n4 == len(_exact_pass_table(N, 4)), N
-        assert exact_twiddles(N).size == 2 * N + 2 * (n8 + n4)
+        assert n16 == len(_exact_pass_table(N, 16)), N
+        assert exact_twiddles(N).size == 2 * N + 2 * (n8 + n4 + n16)
     tw = {N: exact_twiddles(N).view(np.float32) for N in sizes}
     for N, E, Ns, R, off in offs:
-        base = 4 * N + 4 * off           # offsets of the E = 4 table include the E = 8 one
-        tab = tw[N][base: base + 4 * Ns * (R - 1)].reshape(Ns, R - 1, 4).astype(np.float64)
-        k = np.arange(Ns)[:, None]
-        rr = np.arange(1, R)[None, :]
+        base = 4 * N + 4 * off           # offsets include the tables stored before E's
+        tab = tw[N][base: base + 4 * Ns * (R - 1)].reshape(R - 1, Ns, 4).astype(np.float64)
+        k = np.arange(Ns)[None, :]
+        rr = np.arange(1, R)[:, None]
         z = np.exp(-2j * np.pi * k * rr / (Ns * R))
         assert np.abs(tab[..., 0] + tab[..., 2] - z.real).max() < 1e-14
         assert np.abs(tab[..., 1] + tab[..., 3] - z.imag).max() < 1e-14
