@@ -52,8 +52,10 @@ FP_DEV Step exp_step(double x, double S, double L, double dt) {
   const double Ldt = L * dt;
   s.E = exp(Ldt);
   if (fabs(Ldt) > 1e-6) {
-    s.phi1 = (s.E - 1.0) / Ldt;
-    s.dphi1 = (s.E - s.phi1) / Ldt;
+    // one division for both quotients (each then within an ulp of the quotient)
+    const double r = 1.0 / Ldt;
+    s.phi1 = (s.E - 1.0) * r;
+    s.dphi1 = (s.E - s.phi1) * r;
   } else {
     s.phi1 = 1.0 + Ldt * (0.5 + Ldt * (1.0 / 6.0 + Ldt / 24.0));
     s.dphi1 = 0.5 + Ldt * (1.0 / 3.0 + Ldt * 0.125);
@@ -90,16 +92,18 @@ FP_DEV Map compose(const Map& later, const Map& earlier) {
 
 // One step of the recurrence from (e, s); if TAN, also multiplies the running
 // tangent J (row-major 2x2) by the step Jacobian.
+// inv_na = 1 / n_a: the ratios n / n_a are products with it (within an ulp of the quotient).
 template <bool STE, bool TAN>
-FP_DEV St rate_step(St x, double Ia, double Wa, double Wsa, const fp_plasma_params& p, Map& J) {
+FP_DEV St rate_step(St x, double Ia, double Wa, double Wsa, const fp_plasma_params& p,
+                    double inv_na, Map& J) {
   const double na = p.na, dt = p.dt;
-  const double draw = 1.0 - x.e / na;
+  const double draw = 1.0 - x.e * inv_na;
   double depl = draw;
   if (depl < 0.0) depl = 0.0;
   if (depl > 1.0) depl = 1.0;
-  const double dd = (draw > 0.0 && draw < 1.0) ? -1.0 / na : 0.0;
+  const double dd = (draw > 0.0 && draw < 1.0) ? -inv_na : 0.0;
   const double wst = STE ? (Wsa + p.beta_s * Ia * x.e) : 0.0;
-  const double Se = Wa * depl + (STE ? wst * (x.s / na) : 0.0);
+  const double Se = Wa * depl + (STE ? wst * (x.s * inv_na) : 0.0);
   const double Le = p.beta_g * Ia * depl - p.inv_tau_r;
   const Step se = exp_step(x.e, Se, Le, dt);
   St y;
@@ -110,7 +114,7 @@ FP_DEV St rate_step(St x, double Ia, double Wa, double Wsa, const fp_plasma_para
   bool cs = false;
   if (STE) {
     Ss = p.inv_tau_r * x.e;
-    Ls = -wst / na - p.inv_tau_ste;
+    Ls = -wst * inv_na - p.inv_tau_ste;
     ss = exp_step(x.s, Ss, Ls, dt);
     y.s = (!fp_isfinite(ss.xnew) || ss.xnew < 0.0) ? 0.0 : ss.xnew;
     cs = (y.s != ss.xnew) || !fp_isfinite(ss.xnew);
@@ -121,11 +125,11 @@ FP_DEV St rate_step(St x, double Ia, double Wa, double Wsa, const fp_plasma_para
   double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
   if (TAN) {
     const double dLe_de = p.beta_g * Ia * dd;
-    const double dSe_de = Wa * dd + (STE ? p.beta_s * Ia * (x.s / na) : 0.0);
+    const double dSe_de = Wa * dd + (STE ? p.beta_s * Ia * (x.s * inv_na) : 0.0);
     j00 = se.E + x.e * se.E * dt * dLe_de + dt * (dSe_de * se.phi1 + Se * se.dphi1 * dt * dLe_de);
     if (STE) {
-      j01 = dt * (wst / na) * se.phi1;
-      const double dLs_de = -p.beta_s * Ia / na;
+      j01 = dt * (wst * inv_na) * se.phi1;
+      const double dLs_de = -p.beta_s * Ia * inv_na;
       j10 = x.s * ss.E * dt * dLs_de + dt * (p.inv_tau_r * ss.phi1 + Ss * ss.dphi1 * dt * dLs_de);
       j11 = ss.E;
     }
@@ -136,9 +140,9 @@ FP_DEV St rate_step(St x, double Ia, double Wa, double Wsa, const fp_plasma_para
   if (tot > na) {
     const double scale = na / tot;
     if (TAN) {
-      const double T2 = tot * tot;
-      const double r00 = na * y.s / T2, r01 = -na * y.e / T2;
-      const double r10 = -na * y.s / T2, r11 = na * y.e / T2;
+      const double f = scale / tot;
+      const double r00 = f * y.s, r01 = -f * y.e;
+      const double r10 = -f * y.s, r11 = f * y.e;
       const double n00 = r00 * j00 + r01 * j10, n01 = r00 * j01 + r01 * j11;
       const double n10 = r10 * j00 + r11 * j10, n11 = r10 * j01 + r11 * j11;
       j00 = n00; j01 = n01; j10 = n10; j11 = n11;
@@ -181,6 +185,7 @@ FP_DEV int plasma_row(double* smem, const fp_plasma_params& p, const double* __r
   const int lane = tid & 63, wave = tid >> 6;
   const int k0 = tid * C;
   const int nsteps = (tid == TPB - 1) ? C - 1 : C;   // steps k0 .. k0+nsteps-1
+  const double inv_na = 1.0 / p.na;
 
   double Ia[C], Wa[C], Wsa[C];
   {
@@ -228,7 +233,7 @@ FP_DEV int plasma_row(double* smem, const fp_plasma_params& p, const double* __r
 #pragma unroll
     for (int i = 0; i < C; ++i) {
       if (i < nsteps) {
-        x = rate_step<STE, true>(x, Ia[i], Wa[i], Wsa[i], p, J);
+        x = rate_step<STE, true>(x, Ia[i], Wa[i], Wsa[i], p, inv_na, J);
         sI[k0 + i + 1] = x.e;
         if (STE) sS[k0 + i + 1] = x.s;
       }
