@@ -228,9 +228,17 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q + r * Q] = a[r];
     } else {
+      // idxD and r Ns occupy disjoint bits, and lds_pad XORs bits 3..7 into bits 0..4, so
+      // lds_pad(idxD + r Ns) = lds_pad(idxD) ^ lds_pad(r Ns): one swizzled base per
+      // butterfly and one XOR with a constant per output
+      // (byte offsets: the element size is a power of two, so the XOR commutes with it)
       const int idxD = (j / Ns) * Ns * R + k;
+      constexpr unsigned ES = sizeof(cpx<T>);
+      const unsigned bD = (unsigned)lds_pad(idxD) * ES;
+      char* lb = reinterpret_cast<char*>(lds);
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lds_pad(idxD + r * Ns)] = a[r];
+      for (int r = 0; r < R; ++r)
+        *reinterpret_cast<cpx<T>*>(lb + (bD ^ ((unsigned)lds_pad(r * Ns) * ES))) = a[r];
     }
   }
   if constexpr (!LAST) {
