@@ -86,6 +86,83 @@ def test_split_c64_vs_oracle():
     assert rel_l2(al.cpu().numpy(), a_o) < 3e-5
 
 
+@pytest.mark.parametrize("Nt", [2048, 4096])
+def test_nl_step_c64_wide_rows_vs_oracle(Nt):
+    """The FP32 fused nonlinear step (fp_nl_step_c64) at the wide row shapes: 4096 samples
+    run on 256 threads of 16 samples (three radix-16 passes per transform), 2048 samples on
+    256 threads of 8. (a) Against the oracle's FP64 step (alpha/2, RK4 of split(), alpha/2)
+    from the same complex64 spectral input on 8 ionising rows (about 1 rad of nonlinear phase
+    per step): the row within 2e-5 and the step's increment within 3e-5 (measured 1.2e-5 and
+    9e-6 at 4096: FP32 rounding of 18 transforms, the PCHIP and the RK4 sums). (b) With every term
+    off the step is the round trip of its two outer transforms: the coherent gain
+    Re<in, out> / <in, in> - 1 stays below 1e-8 (measured 5e-10 and 1e-9: the exact
+    transforms; the plain float transform, with rounded rotation constants and twiddle powers
+    formed by products, shrinks the field by about 1e-7 per step)."""
+    import rapport_amd as R
+    from rapport_amd import _lib
+    from rapport_amd.mixed import MixedTables
+    from rapport_amd.operators import DeviceTables
+    from scipy.special import erf
+    _announce(f"nl_step c64 Nt={Nt} vs oracle, round-trip gain")
+    kw = cfg_kwargs(C1, Nt=Nt, N=9)
+    cfg_p, cfg_o = R.Config(**kw), O.Config(**kw)
+    g_p, g_o = R.build_grids(cfg_p), O.build_grids(cfg_o)
+    t64 = DeviceTables(cfg_p, g_p)
+    mt = MixedTables(t64, cfg_p, g_p)
+    Nr = g_p["Y"].shape[0]
+    assert Nr == 8
+    # chirped 60 fs Gaussians with 1 % complex noise, band-limited by the spectral mask as the
+    # propagator's fields are (white noise up to the Nyquist frequency, or a random phase per
+    # sample, would make the oracle's own step unstable at this dz)
+    t = (np.arange(Nt) - Nt / 2) * g_p["dt"]
+    rng = np.random.default_rng(5)
+    peaks = np.linspace(2e12, 3e13, Nr)[:, None]
+    noise = 1 + 0.01 * (rng.standard_normal((Nr, Nt)) + 1j * rng.standard_normal((Nr, Nt)))
+    u = np.sqrt(peaks / g_p["invE2"]) * np.exp(-(t / 60e-15) ** 2 + 1j * (t / 40e-15) ** 2) * noise
+    spec = (np.fft.fft(u, axis=1) * g_o["spec_mask"][None, :]).astype(np.complex64)
+    u = np.fft.ifft(spec.astype(np.complex128), axis=1)   # the oracle starts from the same input
+    # a smooth density ramp (a density drawn per sample would also be white noise)
+    rho = np.linspace(1e19, 3e20, Nr)[:, None] * 0.5 * (1 + erf(t / 50e-15))[None, :]
+    nl_o = O.NonlinearOperator(cfg_o, g_o)
+    dz = cfg_o.dz
+    v = u * np.exp(-0.5 * dz * nl_o.alpha_only(u, rho))
+    k1, _ = nl_o.split(v, rho, None)
+    k2, _ = nl_o.split(v + 0.5 * dz * k1, rho, None)
+    k3, _ = nl_o.split(v + 0.5 * dz * k2, rho, None)
+    k4, _ = nl_o.split(v + dz * k3, rho, None)
+    v = v + dz / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    ref = v * np.exp(-0.5 * dz * nl_o.alpha_only(v, rho))
+
+    sd = _cuda(spec)
+    out = torch.empty_like(sd)
+    p = t64.nl_params(dz)
+    _lib.call("fp_nl_step_c64", _lib.ptr(sd), _lib.ptr(out), _lib.ptr(_cuda(rho)), None, 1, 1,
+              ctypes.byref(p), ctypes.byref(mt.tables), None, _lib.stream_ptr())
+    got = np.fft.ifft(out.cpu().numpy().astype(np.complex128), axis=1)
+    e_row, e_inc = rel_l2(got, ref), rel_l2(got - u, ref - u)
+    print(f"[nl c64 Nt={Nt}] rel-L2 row {e_row:.2e} increment {e_inc:.2e} "
+          f"(increment / row {rel_l2(ref, u):.2e})", flush=True)
+    assert rel_l2(ref, u) > 0.1           # the step changes the rows (about 1 rad of phase)
+    assert e_row < 2e-5 and e_inc < 3e-5
+
+    q = t64.nl_params(dz)
+    q.en_kerr_inst = q.en_kerr_raman = q.en_pi_loss = q.en_plasma_absorb = 0
+    q.pc_re = q.pc_im = q.ste_pref = 0.0
+    rng = np.random.default_rng(6)
+    x = (rng.standard_normal((Nr, Nt)) + 1j * rng.standard_normal((Nr, Nt))).astype(np.complex64)
+    xd = _cuda(x)
+    y = torch.empty_like(xd)
+    _lib.call("fp_nl_step_c64", _lib.ptr(xd), _lib.ptr(y), _lib.ptr(_cuda(rho)), None, 1, 1,
+              ctypes.byref(q), ctypes.byref(mt.tables), None, _lib.stream_ptr())
+    yo = y.cpu().numpy().astype(np.complex128)
+    xo = x.astype(np.complex128)
+    gain = float(np.real(np.vdot(xo, yo)) / np.real(np.vdot(xo, xo)) - 1.0)
+    print(f"[nl c64 Nt={Nt}] round trip rel-L2 {rel_l2(yo, xo):.2e} coherent gain {gain:.2e}",
+          flush=True)
+    assert rel_l2(yo, xo) < 1e-6
+    assert abs(gain) < 1e-8
+
+
 def _runs(case, steps, save_stride, **over):
     import rapport_amd as R
     env = R.envelope_lens_noise(**envelope_args(case))
