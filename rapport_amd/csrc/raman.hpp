@@ -45,6 +45,7 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   constexpr int E = NT / TPB;
   static_assert(E % 2 == 0, "the N/2 shift must stay inside a thread");
   static_assert(TPB % 64 == 0, "whole waves");
+  static_assert((E & (E - 1)) == 0, "samples per thread: a power of two");
   int tid = threadIdx.x & (TPB - 1);
   asm volatile("" : "+v"(tid));
   const int lane = tid & 63, wave = tid >> 6;
@@ -63,10 +64,12 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   __syncthreads();
   T Ic[E];
   cpx<T> B = mk<T>(T(0), T(0));
+  // with E a multiple of 8, (tid E + j) >> 3 = (tid E) >> 3 + (j >> 3): one base per thread
+  const int cb = (E % 8 == 0) ? tid * E + ((tid * E) >> 3) : 0;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int n = tid * E + j;
-    Ic[j] = lf[n + (n >> 3)];
+    Ic[j] = (E % 8 == 0) ? lf[cb + j + (j >> 3)] : lf[n + (n >> 3)];
     B = w * mk<T>(B.x + Ic[j], B.y);          // B = sum_j w^(E-j) I'_j
   }
   // weighted inclusive scan X_t = sum_{u <= t} w^(E (t - u)) B_u
@@ -97,9 +100,12 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   const cpx<T> P0 = cdiv(XT, mk<T>(T(1) - wN.x, -wN.y));
   P = P + wt[E * tid] * P0;                   // + w^(E t) P[0]
   __syncthreads();                            // prefix values read
+  // tid E and j < E occupy disjoint bits (E a power of two): lds_pad of their sum is the
+  // XOR of their lds_pad values
+  const int pb = fpfft::lds_pad(tid * E);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    lds[fpfft::lds_pad(tid * E + j)] = P;
+    lds[pb ^ fpfft::lds_pad(j)] = P;
     P = w * mk<T>(P.x + Ic[j], P.y);
   }
   __syncthreads();
