@@ -20,11 +20,19 @@
 
 namespace fpfft {
 
-// LDS slot of element i of the exchange buffer: an XOR swizzle of the low five
-// bits by bits 3..7 (a bijection on [0, N) for N >= 256) that makes every
-// Stockham write and stride-layout read of a wave free of bank conflicts for
-// 8-byte elements (and 2-way, the minimum, for 16-byte elements).
-FP_DEV constexpr int lds_pad(int i) { return i ^ ((i >> 3) & 31); }
+// LDS slot of element i of the exchange buffer: an XOR swizzle of the low five bits by
+// bits 3..7, and of bit 0 by bit 7 (only the low five bits change, each from higher bits,
+// so it is a bijection on [0, N)). It is linear over XOR: lds_pad(a ^ b) = lds_pad(a) ^
+// lds_pad(b). With the LDS serving 16 lanes of 8-byte accesses (8 of 16-byte ones) per
+// cycle from 32 banks, every Stockham write and stride-layout read of the complex64
+// transforms at 8 or 16 samples per thread, and of the complex128 ones at 8, is free of
+// bank conflicts (checked on the host for every row shape; at 16 samples per thread the
+// writes of 16 lanes span index bits 4..7, which bit 7 -> bit 0 makes distinct).
+__host__ __device__ constexpr inline int lds_pad(int i) {
+  return i ^ ((i >> 3) & 31) ^ ((i >> 7) & 1);
+}
+// The XOR mask lds_pad applies to i (a constant when i is).
+__host__ __device__ constexpr inline int lds_mask(int i) { return lds_pad(i) ^ i; }
 template <int N> struct LdsSize { static constexpr int value = N; };
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
@@ -228,9 +236,9 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q + r * Q] = a[r];
     } else {
-      // idxD and r Ns occupy disjoint bits, and lds_pad XORs bits 3..7 into bits 0..4, so
-      // lds_pad(idxD + r Ns) = lds_pad(idxD) ^ lds_pad(r Ns): one swizzled base per
-      // butterfly and one XOR with a constant per output
+      // idxD and r Ns occupy disjoint bits (so their sum is their XOR) and lds_pad is
+      // linear over XOR, so lds_pad(idxD + r Ns) = lds_pad(idxD) ^ lds_pad(r Ns): one
+      // swizzled base per butterfly and one XOR with a constant per output
       // (byte offsets: the element size is a power of two, so the XOR commutes with it)
       const int idxD = (j / Ns) * Ns * R + k;
       constexpr unsigned ES = sizeof(cpx<T>);
@@ -244,11 +252,12 @@ FP_DEV void passes(cpx<T> (&v)[N / TPB], cpx<T>* lds, const cpx<T>* __restrict__
   if constexpr (!LAST) {
     __syncthreads();
     if constexpr (TPB % 256 == 0) {
-      // s * TPB leaves bits 0..7 of tid + s * TPB alone, and lds_pad only reads and
-      // flips bits 0..7: one swizzled base per thread, the rest are immediate offsets
+      // lds_pad(tid + s TPB) = (lds_pad(tid) ^ lds_mask(s TPB)) + s TPB, since lds_pad(tid)
+      // and the mask stay below TPB and s TPB has no bits below it; lds_pad reads no bit
+      // above 7, so the mask is 0: one swizzled base per thread, the rest immediate offsets
       const int b = lds_pad(tid);
 #pragma unroll
-      for (int s = 0; s < E; ++s) v[s] = lds[b + s * TPB];
+      for (int s = 0; s < E; ++s) v[s] = lds[(b ^ lds_mask(s * TPB)) + s * TPB];
     } else {
 #pragma unroll
       for (int s = 0; s < E; ++s) v[s] = lds[lds_pad(tid + s * TPB)];

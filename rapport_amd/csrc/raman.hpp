@@ -100,8 +100,8 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   const cpx<T> P0 = cdiv(XT, mk<T>(T(1) - wN.x, -wN.y));
   P = P + wt[E * tid] * P0;                   // + w^(E t) P[0]
   __syncthreads();                            // prefix values read
-  // tid E and j < E occupy disjoint bits (E a power of two): lds_pad of their sum is the
-  // XOR of their lds_pad values
+  // tid E and j < E occupy disjoint bits (E a power of two), so their sum is their XOR,
+  // and lds_pad is linear over XOR
   const int pb = fpfft::lds_pad(tid * E);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -111,9 +111,9 @@ FP_DEV void raman_row(const cpx<T> (&us)[NT / TPB], T (&KR)[NT / TPB], cpx<T>* l
   __syncthreads();
   cpx<T> Ps[E];
   if constexpr (TPB % 256 == 0) {
-    const int b = fpfft::lds_pad(tid);          // s * TPB does not touch the swizzled bits
+    const int b = fpfft::lds_pad(tid);          // as in the transforms' stride-layout reads
 #pragma unroll
-    for (int s = 0; s < E; ++s) Ps[s] = lds[b + s * TPB];
+    for (int s = 0; s < E; ++s) Ps[s] = lds[(b ^ fpfft::lds_mask(s * TPB)) + s * TPB];
   } else {
 #pragma unroll
     for (int s = 0; s < E; ++s) Ps[s] = lds[fpfft::lds_pad(tid + s * TPB)];
