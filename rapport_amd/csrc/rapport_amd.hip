@@ -271,8 +271,7 @@ template <int NT, int TPB, bool STE, bool PERS>
 __global__ __launch_bounds__(TPB, (PlasmaOcc<float, STE>::value)) void batch_plasma_kernel(const fp_batch_instance* __restrict__ insts,
                                                            BatchGroup g) {
   __shared__ __attribute__((aligned(16))) char smem[PlasmaSmem<float, NT, TPB, STE>::bytes];
-  const int total = PERS ? g.begin[g.n] : (int)blockIdx.x + 1;
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+  auto one = [&](int item) {
     const int j = group_find(g, item);
     const fp_batch_instance& I = insts[__builtin_amdgcn_readfirstlane(g.idx[j])];
     const int row = item - g.begin[j];
@@ -289,7 +288,15 @@ __global__ __launch_bounds__(TPB, (PlasmaOcc<float, STE>::value)) void batch_pla
     if (row == 0 && threadIdx.x < 3) slots[1 + threadIdx.x] = 0u;
     plasma_row_body<float, NT, TPB, STE, true>(smem, row, spec, rho, rho_s, nullptr, pp, W, Ws,
                                                tw, status);
-    if (PERS) __syncthreads();
+  };
+  if constexpr (PERS) {
+    const int total = g.begin[g.n];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+      one(item);
+      __syncthreads();
+    }
+  } else {
+    one(blockIdx.x);
   }
 }
 
@@ -297,14 +304,22 @@ template <int NT, int TPB, bool RS, bool PERS>
 __global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void batch_nl_kernel(
     const fp_batch_instance* __restrict__ insts, BatchGroup g) {
   __shared__ fpnl32::StepSmem<NT> sm;
-  const int total = PERS ? g.begin[g.n] : (int)blockIdx.x + 1;
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+  auto one = [&](int item) {
     const int j = group_find(g, item);
     const fp_batch_instance& I = insts[g.idx[j]];
     const int row = item - g.begin[j];
     fpnl32::nl_step_row<NT, TPB, RS>(sm, row, I.w0, I.w0, I.rho, RS ? I.rho_s : nullptr, 1, 1,
                                      I.np, I.nt, I.slots + 2);
-    if (PERS) __syncthreads();
+  };
+  if constexpr (PERS) {
+    const int total = g.begin[g.n];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+      one(item);
+      __syncthreads();
+    }
+  } else {
+    one(blockIdx.x);   // one block per item: no loop state (the 4096-sample rows then need
+                       // fewer registers)
   }
 }
 
