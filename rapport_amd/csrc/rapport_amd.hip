@@ -1430,12 +1430,15 @@ int fp_adj_nl_step_c64(const float* v1s, const double* rho, const double* rho_s,
   if (p->Nr <= 0) return fail(FP_E_ARG, "fp_adj_nl_step_c64: bad Nr");
   if (pcoef_misaligned(t)) return fail(FP_E_ARG, "fp_adj_nl_step_c64: pcoef_s must be 16-byte aligned");
   hipStream_t st = S(stream);
+  // Nt / 8 threads of 8 samples (222-234 VGPRs, two waves per SIMD, no spills): at
+  // 512 x 2048 the backward sweep takes 21 % less time than with Nt / 4 threads of 4
+  // samples (six radix-4 passes per transform instead of four radix-8 ones)
 #define FP_ADJNL(NT)                                                                           \
   if (rho_s)                                                                                   \
-    hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 4, true>), dim3(p->Nr), dim3(NT / 4), 0, \
+    hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 8, true>), dim3(p->Nr), dim3(NT / 8), 0, \
                        st, v1s, rho, rho_s, g4s, g1s, grho, grho_s, scratch, *p, *t, cmax_out); \
   else                                                                                         \
-    hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 4, false>), dim3(p->Nr), dim3(NT / 4),   \
+    hipLaunchKernelGGL((fpadj::adj_nl_kernel<NT, NT / 8, false>), dim3(p->Nr), dim3(NT / 8),   \
                        0, st, v1s, rho, nullptr, g4s, g1s, grho, nullptr, scratch, *p, *t,     \
                        cmax_out);                                                              \
   return check_launch("adj_nl_step_c64");
