@@ -232,11 +232,15 @@ int dispatch_plasma(const T* src, double* rho, double* rhos, T* u_out, const fp_
 }
 
 // ------------------------------------------------------------------ nonlinear
-template <int NT> struct NLT { static constexpr int TPB = NT >= 4096 ? NT / 8 : NT / 4; };
-// FP32 rows of 4096 samples: 256 threads of 16 samples (three radix-16 passes per
-// transform, 80 KiB of LDS, two blocks per CU) take 12 % less time than 512 threads
-// of 8 samples (four radix-8 passes) at 1024 x 4096.
-template <int NT> struct NLT32 { static constexpr int TPB = NT >= 4096 ? NT / 16 : NT / 8; };
+// Samples per thread of the nonlinear-step rows, from measured step times: FP64 rows take 8
+// from 2048 samples on (512 x 2048: 0.471 vs 0.529 ms per step with 4) and 4 below (more
+// waves for the small grids); FP32 rows of 4096 samples take 16 (256 threads, three radix-16
+// passes per transform, 80 KiB of LDS, two blocks per CU: 12 % less time than 8 samples at
+// 1024 x 4096), 8 at 2048 and 4 below (128 x 512: 0.0876 vs 0.0935 ms per mixed step with 8).
+template <int NT> struct NLT { static constexpr int TPB = NT >= 2048 ? NT / 8 : NT / 4; };
+template <int NT> struct NLT32 {
+  static constexpr int TPB = NT >= 4096 ? NT / 16 : (NT >= 2048 ? NT / 8 : NT / 4);
+};
 
 // ------------------------------------------------------------------ batched form
 // Work items (rows of the row kernels, output tiles of the GEMM) of every instance
