@@ -1083,6 +1083,29 @@ int fp_plasma_scan_spec_c64(const float* spec, double* rho, double* rho_s, float
                                       clear_slots, S(stream));
 }
 
+}  // extern "C"
+
+namespace {
+
+// Compute units of the current device (cached per device; 256 if the query fails).
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
 int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const float* B,
                          float* C, int Nr, int Nt, float alpha, const float* rowscale,
                          const float* phase, const unsigned* bmax_in, unsigned* cmax_out,
@@ -1092,23 +1115,27 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
   const int Ncols = 2 * Nt;
   if (Nr <= 0 || Nr % 128 || Ncols % 128)
     return fail(FP_E_SHAPE, "fp_hankel_gemm_mixed: needs Nr % 128 == 0 and 2 Nt % 128 == 0");
-  // 256 x 128 tiles of 8 waves (4 x 2 of 64 x 64), one block per CU, when Nr allows: 5 %
-  // faster than 128 x 128 tiles of 4 waves at 1024 x 4096 (each B panel is staged by half
-  // as many blocks), results bitwise equal; else 128 x 128, two blocks per CU. (A 128 x 256
-  // tile of 4 waves at one block per CU measured slower: 85 vs 66 us.)
+  // Tile by the number of tiles against the CUs (every tile shape gives each output the same
+  // K order, so the results are bitwise equal). 256 x 128 tiles of 8 waves, one block per CU,
+  // when they fill the CUs: 5 % faster than 128 x 128 tiles at 1024 x 4096 (each B panel is
+  // staged by half as many blocks). Smaller problems take tiles of 4 waves, two blocks per CU:
+  // 128 x 128 when they fill both slots of every CU, else 128 x 64 when they fill the CUs,
+  // else 64 x 64 (mixed step 512 x 2048: 0.194 -> 0.158 ms, 256 x 1024: 0.120 -> 0.090 ms,
+  // 128 x 512: 0.088 -> 0.069 ms against 256 x 128 / 128 x 128 tiles).
   const auto* yh = reinterpret_cast<const fpgemm16::h8*>(Yh);
   const auto* yl = reinterpret_cast<const fpgemm16::h8*>(Yl);
   constexpr int NPF = FP_MIXED_GEMM_NPF;
-  if (Nr % 256 == 0)
-    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<256, 128, 4, 2, NPF>),
-                       dim3((Nr / 256) * (Ncols / 128)), dim3(512), 0, S(stream), yh, yl,
-                       y_scale_exp, B, C, Nr, Nr, Ncols, alpha, rowscale, phase, bmax_in, cmax_out,
-                       reset_slot);
-  else
-    hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<128, 128, 2, 2, NPF>),
-                       dim3((Nr / 128) * (Ncols / 128)),
-                       dim3(256), 0, S(stream), yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols, alpha,
-                       rowscale, phase, bmax_in, cmax_out, reset_slot);
+  const int cus = device_cus();
+  auto tiles = [&](int bm, int bn) { return (Nr / bm) * (Ncols / bn); };
+#define FP_MIXED_TILE(BM, BN, WM, WN)                                                          \
+  hipLaunchKernelGGL((fpgemm16::hankel_gemm_f16x3<BM, BN, WM, WN, NPF>), dim3(tiles(BM, BN)),  \
+                     dim3(WM * WN * 64), 0, S(stream), yh, yl, y_scale_exp, B, C, Nr, Nr, Ncols, \
+                     alpha, rowscale, phase, bmax_in, cmax_out, reset_slot)
+  if (Nr % 256 == 0 && tiles(256, 128) >= cus) FP_MIXED_TILE(256, 128, 4, 2);
+  else if (tiles(128, 128) >= 2 * cus) FP_MIXED_TILE(128, 128, 2, 2);
+  else if (tiles(128, 64) >= cus) FP_MIXED_TILE(128, 64, 2, 2);
+  else FP_MIXED_TILE(64, 64, 2, 2);
+#undef FP_MIXED_TILE
   return check_launch("hankel_gemm_mixed");
 }
 
@@ -1127,20 +1154,6 @@ struct BandPlan {
   fpgemm16::ColBand band;
   int zlo, zhi;   // real columns left to zero
 };
-
-int device_cus() {
-  static int cus[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cus[dev]) {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        v <= 0)
-      v = 256;
-    cus[dev] = v;
-  }
-  return cus[dev];
-}
 
 // tile_rows x (TBK 16-column blocks) tiles, `per_cu` blocks resident per CU
 bool plan_band(int Nr, int Ncols, int zc_lo, int zc_hi, int cus, BandPlan* bp,
