@@ -263,8 +263,9 @@ int fp_hankel_gemm_mixed(const void* Yh, const void* Yl, int y_scale_exp, const 
 /* fp_hankel_gemm restricted to the columns outside [zc_lo, zc_hi) (real columns), where
  * the caller knows the product is exactly zero (the spectral mask is 0 there): the other
  * columns are tiled to fill the GPU once and are bitwise fp_hankel_gemm's values; columns in
- * the range are not written, or, with zero_fill, set to 0. Falls back to fp_hankel_gemm
- * when the range is too narrow to gain anything or Nr % 128 != 0. Replaces
+ * the range are not written, or, with zero_fill, set to 0. An empty or too narrow range
+ * (zc_lo = zc_hi = 0) tiles all columns the same way when 128 x 128 tiles would not fill
+ * the GPU. Falls back to fp_hankel_gemm otherwise, or when Nr % 128 != 0. Replaces
  * cp.dot(self.Y, u) (D3/sim/operators.py:48,53) in the z-step pipeline. */
 int fp_hankel_gemm_band(const double* Yt, const double* B, double* C, int Nr, int Nt,
                         double alpha, const double* rowscale, const double* phase, int zc_lo,

@@ -1155,17 +1155,28 @@ struct BandPlan {
   int zlo, zhi;   // real columns left to zero
 };
 
-// tile_rows x (TBK 16-column blocks) tiles, `per_cu` blocks resident per CU
+// tile_rows x (TBK 16-column blocks) tiles, `per_cu` blocks resident per CU, TBK in
+// [min_tbk, 7]. With whole_width (and no usable zero range) the tiles cover all columns:
+// TBK is then lowered to a divisor of the column blocks, since there is no masked range to
+// take padding blocks from.
 bool plan_band(int Nr, int Ncols, int zc_lo, int zc_hi, int cus, BandPlan* bp,
-               int tile_rows = 256, int per_cu = 1) {
-  if (Nr % tile_rows || zc_lo < 0 || zc_hi > Ncols || zc_hi <= zc_lo) return false;
-  const int a16 = (zc_lo + 15) / 16 * 16, b16 = zc_hi / 16 * 16;
-  if (b16 - a16 < 16) return false;
+               int tile_rows = 256, int per_cu = 1, int min_tbk = 4, bool whole_width = false) {
+  if (Nr % tile_rows || Ncols % 16) return false;
+  int a16 = Ncols, b16 = Ncols;
+  if (zc_lo >= 0 && zc_hi <= Ncols && zc_hi > zc_lo) {
+    a16 = (zc_lo + 15) / 16 * 16;
+    b16 = zc_hi / 16 * 16;
+  }
+  const bool banded = b16 - a16 >= 16;
+  if (!banded && !whole_width) return false;
+  if (!banded) a16 = b16 = Ncols;
   const int Lb = (a16 + (Ncols - b16)) / 16;
   const int per_panel = cus * per_cu / (Nr / tile_rows);
   if (per_panel < 1 || Lb < 1) return false;
-  const int tbk = (Lb + per_panel - 1) / per_panel;
-  if (tbk < 4 || tbk > 7) return false;
+  int tbk = (Lb + per_panel - 1) / per_panel;
+  if (tbk < min_tbk || tbk > 7) return false;
+  if (!banded)
+    while (Lb % tbk) --tbk;
   const int slots = (Lb + tbk - 1) / tbk;
   const int va = a16 + (slots * tbk - Lb) * 16;
   if (va > b16) return false;
@@ -1190,7 +1201,11 @@ int fp_hankel_gemm_band(const double* Yt, const double* B, double* C, int Nr, in
   if (zc_lo < 0 || zc_hi > Ncols || (zc_hi > 0 && zc_hi < zc_lo))
     return fail(FP_E_ARG, "fp_hankel_gemm_band: zero-column range outside [0, 2 Nt)");
   BandPlan bp;
-  if (!plan_band(Nr, Ncols, zc_lo, zc_hi, device_cus(), &bp, fpgemm::BM, 2))
+  // 128-row tiles of TBK = 1 .. 7 blocks, also over the whole width when there is no zero
+  // range: the band kernel's per-element arithmetic is hankel_gemm_f64's, and narrow tiles
+  // fill the CUs at sizes where 128 x 128 tiles do not (512 x 2048: 128 tiles for 512 block
+  // slots)
+  if (!plan_band(Nr, Ncols, zc_lo, zc_hi, device_cus(), &bp, fpgemm::BM, 2, 1, true))
     return fp_hankel_gemm(Yt, B, C, Nr, Nt, alpha, rowscale, phase, stream);
   const int zlo = zero_fill ? bp.zlo : 0, zhi = zero_fill ? bp.zhi : 0;
   const dim3 grid((Nr / fpgemm::BM) * bp.slots);
@@ -1200,6 +1215,9 @@ int fp_hankel_gemm_band(const double* Yt, const double* B, double* C, int Nr, in
                      Yt, B, C, Nr, Nr, Ncols, alpha, rowscale, phase, bp.slots, band, zlo, zhi); \
   break;
   switch (bp.tbk) {
+    case 1: FP_BAND64(1)
+    case 2: FP_BAND64(2)
+    case 3: FP_BAND64(3)
     case 4: FP_BAND64(4)
     case 5: FP_BAND64(5)
     case 6: FP_BAND64(6)
