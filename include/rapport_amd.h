@@ -74,10 +74,14 @@ typedef struct fp_nl_tables {
  * in the local coordinate s = (I - x_i)/h_i and pre-scaled by photo_pref, so
  * photo = |P(s)| * depl / (|u|^2 + 1e-30) stays inside float range. */
 typedef struct fp_nl_tables_f32 {
-  const float* tw;        /* 2 Nt complex64: twiddles tw[m] = exp(-2 pi i m / Nt) rounded,
-                             then their residuals tw[Nt + m] = exp(-2 pi i m / Nt) - tw[m]
-                             (read by the nonlinear step's transforms into and out of the
-                             row, which have no coherent rounding gain) */
+  const float* tw;        /* complex64: twiddles tw[m] = exp(-2 pi i m / Nt) rounded (m < Nt),
+                             their residuals tw[Nt + m] = exp(-2 pi i m / Nt) - tw[m], then
+                             from tw[2 Nt] the compact per-pass tables of the exact
+                             transforms (rows of 8, 4 and 16 samples per thread; per pass,
+                             power-major, the pair (w^r rounded, residual) as 4 floats), read
+                             by the nonlinear step's transforms into and out of the row, which
+                             have no coherent rounding gain; rapport_amd.mixed.exact_twiddles
+                             builds it */
   const float* mA;
   const float* mB;
   const float* mC;
