@@ -41,13 +41,21 @@ __global__ __launch_bounds__(TPB) void record_history_kernel(
   const cpx<T>* sp = reinterpret_cast<const cpx<T>*>(spec) + row * NT;
 #pragma unroll
   for (int s = 0; s < E; ++s) v[s] = sp[tid + s * TPB];
+  // the densities are loaded before the transform, so their latency overlaps it (loads are
+  // not moved across the transform's barriers)
+  const double* rr_row = rho + row * NT;
+  const double* rs_row = rhos ? rhos + row * NT : nullptr;
+  double rrv[E], rsv[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    rrv[s] = rr_row[tid + s * TPB];
+    rsv[s] = rs_row ? rs_row[tid + s * TPB] : 0.0;
+  }
   fpfft::row_fft<T, NT, TPB, true>(v, lds, reinterpret_cast<const cpx<T>*>(tw));
   const T sc = T(1) / (T)NT;
 
   double vs[7] = {0.0, -INFINITY, -INFINITY, -INFINITY, 0.0, 0.0, 0.0};
   double dep[E];
-  const double* rr_row = rho + row * NT;
-  const double* rs_row = rhos ? rhos + row * NT : nullptr;
   const bool cube_row = d.t_stride > 0 && (int)(row % (size_t)d.r_stride) == 0;
   const size_t cube_off = cube_row ? (row / d.r_stride) * (size_t)d.Nt_sub : 0;
 #pragma unroll
@@ -56,8 +64,8 @@ __global__ __launch_bounds__(TPB) void record_history_kernel(
     const cpx<T> w = sc * v[s];
     const double x = (double)w.x, y = (double)w.y;
     const double a2 = x * x + y * y;
-    const double rr = rr_row[i];
-    const double rs = rs_row ? rs_row[i] : 0.0;
+    const double rr = rrv[s];
+    const double rs = rsv[s];
     const double I = a2 * p.invE2;
     double ph = 0.0;
     if (p.en_pi_loss) {
