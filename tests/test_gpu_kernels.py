@@ -119,15 +119,19 @@ def test_hankel_gemm(Nr, Nt):
 
 
 @pytest.mark.parametrize("Nr,Nt,zc,expect_band", [(1024, 4096, (4096, 6048), True),
-                                                  (512, 2048, (1000, 2900), False),
+                                                  (512, 2048, (1000, 2900), True),
                                                   (768, 4096, (3500, 5000), True),
-                                                  (200, 512, (300, 700), False)])
+                                                  (200, 512, (300, 700), False),
+                                                  (512, 2048, (0, 0), False),
+                                                  (128, 512, (0, 0), False)])
 def test_hankel_gemm_band_equals_full(Nr, Nt, zc, expect_band):
     # the banded FP64 GEMM (tiles only over the columns outside the masked range [zc)) gives
     # bitwise fp_hankel_gemm's values there, with and without phase / row scale; zero_fill
     # writes 0 in the masked columns, otherwise they keep their contents (16-column
-    # alignment: the range's ends are computed). Nr = 200 is not tiled by 128 rows: the entry
-    # falls back to the full GEMM.
+    # alignment: the range's ends are computed). At 512 x 2048 the tiles are 2 column blocks
+    # wide; with no range (0, 0) the whole width is tiled in tiles of 1 .. 4 column blocks
+    # where 128 x 128 tiles would not fill the GPU. Nr = 200 is not tiled by 128 rows: the
+    # entry falls back to the full GEMM.
     from rapport_amd import _lib
     _announce(f"hankel_gemm_band Nr={Nr} Nt={Nt} zero columns {zc}")
     rng = np.random.default_rng(Nr + 1)
