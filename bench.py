@@ -333,6 +333,10 @@ def run_batched(n_inst=64, steps=20, seed=0, tune=True):
     # the same sweep with one launch per pass (six per z-step)
     bm = BatchedIntegrator(cfgs, envs)
     bm.set_merged(True)
+    passes_m = {}
+    for name, p in (("plasma", 0), ("G1", 1), ("G2", 2), ("nl", 3), ("G3", 4), ("G4", 5)):
+        passes_m[name] = _time(lambda: call("fp_batch_pass", bm._plan, p, st), 1, 3)
+    bm.refresh_scales()
     gm = bm.capture(1)
     for _ in range(2):
         gm.replay()
@@ -343,7 +347,8 @@ def run_batched(n_inst=64, steps=20, seed=0, tune=True):
     torch.cuda.synchronize()
     dtm = (time.perf_counter() - t0) / steps
     out["single_launch_per_pass"] = dict(ms_per_step=1e3 * dtm, inst_steps_per_s=n_inst / dtm,
-                                         launches_per_step=bm.launches_per_step())
+                                         launches_per_step=bm.launches_per_step(),
+                                         pass_ms=passes_m)
     bm.close()
     del b, g, bm, gm
     torch.cuda.empty_cache()

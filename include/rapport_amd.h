@@ -399,10 +399,11 @@ int fp_batch_set_config(fp_batch_plan* plan, int i, int bm, int bn, int nl_tpb);
 int fp_batch_set_grid(fp_batch_plan* plan, int kind, int blocks);
 
 /* merged = 1: every pass is one launch for the whole batch (all Nt classes, STE flags and
- * tile shapes in one work list; 256 x 128 GEMM tiles where Nr % 256 == 0, else 128 x 256;
- * nonlinear step with Nt / 8 threads per row; persistent grids sized to the device), so a
- * z-step is six launches; the per-instance tiles, threads and grids of fp_batch_set_config /
- * fp_batch_set_grid then do not apply. Needs Nr % 128 == 0. merged = 0: one launch per
+ * tile shapes in one work list; one block per work item: a row of the carrier scan, 4096 / Nt
+ * rows of the nonlinear step at Nt / 8 threads per row, a 256 x 128 GEMM tile), so a z-step
+ * is six launches; the per-instance tiles, threads and grids of fp_batch_set_config /
+ * fp_batch_set_grid then do not apply. Needs Nr % 256 == 0 (FP_E_SHAPE otherwise, and the
+ * plan keeps its launch mode). merged = 0: one launch per
  * group of instances sharing a kernel configuration (the default). */
 int fp_batch_set_merged(fp_batch_plan* plan, int merged);
 

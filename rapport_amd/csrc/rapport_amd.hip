@@ -380,9 +380,14 @@ void batch_gemm_kernel(const fp_batch_instance* __restrict__ insts, BatchGroup g
 
 // ------------------------------------------------------------------ merged passes
 // One launch per pass for the whole batch: every instance's rows (or tiles) form one work
-// list, whatever its Nt and STE flag (row kernels) or tile shape (GEMM); a block switches
-// on the item's instance. The heavier instances come first in the list (see
-// rebuild_groups), so a persistent grid ends on the light ones.
+// list, whatever its Nt and STE flag (row kernels); the grid has one block per work item.
+// A block finds its item's instance (uniform: the index is read with readfirstlane, so the
+// descriptor loads are scalar loads), reads every descriptor field it needs before its first
+// store (a later read through the descriptor pointer could alias that store and would become
+// a vector load, with the pointers then held in VGPRs), and switches on the instance's Nt to
+// the inlined row body of that size. All bodies share the kernel's register budget, which is
+// that of the largest; no body is a separate function (a call spills its live registers to
+// scratch memory around it). The heavier instances come first in the list (rebuild_groups).
 template <bool HAS_STE>
 struct MergedPlasmaSmem {
   static constexpr int bytes =
@@ -390,142 +395,122 @@ struct MergedPlasmaSmem {
            cmax(PlasmaSmem<float, 2048, 256, HAS_STE>::bytes, PlasmaSmem<float, 4096, 256, HAS_STE>::bytes));
 };
 
-// The per-variant bodies are separate (noinline) functions, each register-allocated on its
-// own: inlined together behind one switch they spilled. Their LDS arrives as an
-// address_space(3) pointer, so that the accesses stay LDS instructions (a plain pointer
-// argument would make them flat accesses).
-typedef __attribute__((address_space(3))) char lds_char;
-FP_DEV lds_char* to_lds(char* p) { return (lds_char*)p; }
-
-template <int NT, bool STE>
-__device__ __attribute__((noinline)) void plasma_rows_merged(lds_char* smem3,
-                                                              const fp_batch_instance& I, int row) {
-  char* smem = (char*)smem3;
-  plasma_row_body<float, NT, 256, STE, true>(smem, row, I.spec, I.rho, STE ? I.rho_s : nullptr,
-                                             nullptr, I.pp, I.W_lut, I.Ws_lut, I.tw, I.status);
-}
-
 template <bool HAS_STE>
-__global__ __launch_bounds__(256) void batch_plasma_merged(
+__global__ __launch_bounds__(256, (PlasmaOcc<float, HAS_STE>::value)) void batch_plasma_merged(
     const fp_batch_instance* __restrict__ insts, BatchGroup g) {
   __shared__ __attribute__((aligned(16))) char smem[MergedPlasmaSmem<HAS_STE>::bytes];
-  const int total = g.begin[g.n];
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int j = group_find(g, item);
-    const fp_batch_instance& I = insts[g.idx[j]];
-    const int row = item - g.begin[j];
-    if (row == 0 && threadIdx.x < 3) I.slots[1 + threadIdx.x] = 0u;
-    const bool ste = HAS_STE && I.enable_ste;
-#define FP_MPL(NT)                                                     \
-  if (ste) plasma_rows_merged<NT, HAS_STE>(to_lds(smem), I, row);      \
-  else plasma_rows_merged<NT, false>(to_lds(smem), I, row);            \
+  const int item = blockIdx.x;
+  const int j = group_find(g, item);
+  const fp_batch_instance& I = insts[__builtin_amdgcn_readfirstlane(g.idx[j])];
+  const int row = item - g.begin[j];
+  const int Nt = I.Nt;
+  const bool ste = HAS_STE && I.enable_ste;
+  const fp_plasma_params pp = I.pp;
+  const float* spec = I.spec;
+  double *rho = I.rho, *rho_s = HAS_STE ? I.rho_s : nullptr;
+  const double *W = I.W_lut, *Ws = I.Ws_lut;
+  const float* tw = I.tw;
+  int* status = I.status;
+  unsigned* slots = I.slots;
+  if (row == 0 && threadIdx.x < 3) slots[1 + threadIdx.x] = 0u;
+#define FP_MPL(NT)                                                                             \
+  if (ste)                                                                                     \
+    plasma_row_body<float, NT, 256, HAS_STE, true>(smem, row, spec, rho, rho_s, nullptr, pp, W, \
+                                                   Ws, tw, status);                            \
+  else                                                                                         \
+    plasma_row_body<float, NT, 256, false, true>(smem, row, spec, rho, nullptr, nullptr, pp, W, \
+                                                 Ws, tw, status);                              \
   break;
-    switch (I.Nt) {
-      case 512: FP_MPL(512)
-      case 1024: FP_MPL(1024)
-      case 2048: FP_MPL(2048)
-      default: FP_MPL(4096)
-    }
-#undef FP_MPL
-    __syncthreads();
+  switch (Nt) {
+    case 512: FP_MPL(512)
+    case 1024: FP_MPL(1024)
+    case 2048: FP_MPL(2048)
+    default: FP_MPL(4096)
   }
+#undef FP_MPL
 }
 
 // Nonlinear step, 512 threads: an item is 4096 / Nt rows of one instance, one per group of
-// Nt / 8 threads, each with its own LDS region.
+// Nt / 8 threads (eight samples per thread), each group with its own LDS region. 128 VGPRs,
+// two blocks (16 waves) and 160 KiB of LDS per CU.
 template <int NT, bool RS>
-__device__ __attribute__((noinline)) void nl_rows_merged(lds_char* smem3, const fp_batch_instance& I,
-                                                          int grp, const fp_nl_params& np,
-                                                          const fp_nl_tables_f32& nt) {
-  char* smem = (char*)smem3;
+FP_DEV void nl_rows_merged(char* smem, const fp_batch_instance& I, int grp, float* w0,
+                           const double* rho, const double* rho_s, unsigned* cmax) {
   constexpr int TPB = NT / 8, RPB = 512 / TPB;
   const int sub = threadIdx.x / TPB;
   auto& sm = reinterpret_cast<fpnl32::StepSmem<NT>*>(smem)[sub];
-  fpnl32::nl_step_row<NT, TPB, RS>(sm, (size_t)grp * RPB + sub, I.w0, I.w0, I.rho,
-                                   RS ? I.rho_s : nullptr, 1, 1, np, nt, I.slots + 2);
+  fpnl32::nl_step_row<NT, TPB, RS>(sm, (size_t)grp * RPB + sub, w0, w0, rho, RS ? rho_s : nullptr,
+                                   1, 1, I.np, I.nt, cmax);
 }
 
 template <bool HAS_STE>
 __global__ __launch_bounds__(512, 4) void batch_nl_merged(const fp_batch_instance* __restrict__ insts,
                                                           BatchGroup g) {
-  // 80 KiB: two blocks (16 waves) per CU
   __shared__ __attribute__((aligned(16))) char smem[sizeof(fpnl32::StepSmem<4096>)];
   static_assert(8 * sizeof(fpnl32::StepSmem<512>) <= sizeof(fpnl32::StepSmem<4096>), "LDS");
-  const int total = g.begin[g.n];
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int j = group_find(g, item);
-    const fp_batch_instance& I = insts[g.idx[j]];
-    const int grp = item - g.begin[j];
-    const bool ste = HAS_STE && I.enable_ste;
-#define FP_MNL(NT)                                                          \
-  if (ste) nl_rows_merged<NT, HAS_STE>(to_lds(smem), I, grp, I.np, I.nt);    \
-  else nl_rows_merged<NT, false>(to_lds(smem), I, grp, I.np, I.nt);          \
+  const int item = blockIdx.x;
+  const int j = group_find(g, item);
+  const fp_batch_instance& I = insts[__builtin_amdgcn_readfirstlane(g.idx[j])];
+  const int grp = item - g.begin[j];
+  const int Nt = I.Nt;
+  const bool ste = HAS_STE && I.enable_ste;
+  float* w0 = I.w0;
+  const double* rho = I.rho;
+  const double* rho_s = HAS_STE ? I.rho_s : nullptr;
+  unsigned* cmax = I.slots + 2;
+#define FP_MNL(NT)                                                      \
+  if (ste) nl_rows_merged<NT, HAS_STE>(smem, I, grp, w0, rho, rho_s, cmax); \
+  else nl_rows_merged<NT, false>(smem, I, grp, w0, rho, rho_s, cmax);       \
   break;
-    switch (I.Nt) {
-      case 512: FP_MNL(512)
-      case 1024: FP_MNL(1024)
-      case 2048: FP_MNL(2048)
-      default: FP_MNL(4096)
-    }
-#undef FP_MNL
-    __syncthreads();
+  switch (Nt) {
+    case 512: FP_MNL(512)
+    case 1024: FP_MNL(1024)
+    case 2048: FP_MNL(2048)
+    default: FP_MNL(4096)
   }
+#undef FP_MNL
 }
 
-// GEMM stage, 512 threads: 256 x 128 tiles (4 x 2 waves) for Nr % 256 == 0, else 128 x 256
-// (2 x 4 waves); same operands, scale slots and masked-tile skipping as batch_gemm_kernel.
-// noinline: each tile shape is register-allocated on its own (inlined together, the two
-// shapes spilled)
-template <int BM, int BN, int WM, int WN, int NPF, int STAGE>
-__device__ __attribute__((noinline)) void gemm_item_merged(lds_char* smem3, const fp_batch_instance& I,
-                                                            int local) {
-  auto& sm = *reinterpret_cast<fpgemm16::Smem<BM, BN, WM, WN>*>((char*)smem3);
-  const int tilesM = I.Nr / BM;
+// GEMM stage, 512 threads, one 256 x 128 output tile (4 x 2 waves) per block, for instances
+// with Nr % 256 == 0 (fp_batch_set_merged checks it); same operands, scale slots and
+// masked-tile skipping as batch_gemm_kernel. XCD-aware item order (whole column panels of an
+// instance per XCD, so a panel's B rows stay in that XCD's L2).
+template <int STAGE>
+__global__ __launch_bounds__(512, 1) void batch_gemm_merged(const fp_batch_instance* __restrict__ insts,
+                                                            BatchGroup g) {
+  constexpr int BM = 256, BN = 128;
+  __shared__ fpgemm16::Smem<BM, BN, 4, 2> sm;
+  const int total = g.begin[g.n];
+  const int item = fpgemm16::xcd_remap(blockIdx.x, total);
+  const int j = group_find(g, item);
+  const fp_batch_instance& I = insts[__builtin_amdgcn_readfirstlane(g.idx[j])];
+  const int local = item - g.begin[j];
+  const int Nr = I.Nr, Nt = I.Nt, zc_lo = I.zc_lo, zc_hi = I.zc_hi, yse = I.y_scale_exp;
+  const auto* Yh = reinterpret_cast<const fpgemm16::h8*>(I.Yh);
+  const auto* Yl = reinterpret_cast<const fpgemm16::h8*>(I.Yl);
+  //                G1        G2       G3        G4
   const float* B = STAGE == 0 ? I.spec : STAGE == 2 ? I.w0 : I.w1;
   float* C = STAGE == 1 ? I.w0 : STAGE == 3 ? I.spec : I.w1;
   const float alpha = (STAGE & 1) ? I.iRk : I.Rk;
   const float* rs = STAGE == 3 ? I.mask_r : nullptr;
   const float* ph = (STAGE & 1) ? nullptr : I.phase;
-  const unsigned* bin = I.slots + STAGE;
-  unsigned* cout = STAGE == 0 ? I.slots + 1 : STAGE == 2 ? I.slots + 3 : STAGE == 3 ? I.slots : nullptr;
-  if (STAGE == 1 && local == 0 && threadIdx.x == 0) I.slots[0] = 0u;
+  unsigned* slots = I.slots;
+  const unsigned* bin = slots + STAGE;
+  unsigned* cout = STAGE == 0 ? slots + 1 : STAGE == 2 ? slots + 3 : STAGE == 3 ? slots : nullptr;
+  if (STAGE == 1 && local == 0 && threadIdx.x == 0) slots[0] = 0u;
+  const int tilesM = Nr / BM;
   const int tm = local % tilesM, tn = local / tilesM;
-  if (tn * BN >= I.zc_lo && (tn + 1) * BN <= I.zc_hi) {
+  if (tn * BN >= zc_lo && (tn + 1) * BN <= zc_hi) {
     float4* Cz = reinterpret_cast<float4*>(C);
-    const int ld4 = (2 * I.Nt) / 4;
+    const int ld4 = (2 * Nt) / 4;
     for (int i = threadIdx.x; i < BM * BN / 4; i += 512) {
       const int r = i / (BN / 4), c4 = i % (BN / 4);
       Cz[(size_t)(tm * BM + r) * ld4 + (tn * BN) / 4 + c4] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     return;
   }
-  fpgemm16::gemm_tile<BM, BN, WM, WN, NPF>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(I.Yh),
-                                           reinterpret_cast<const fpgemm16::h8*>(I.Yl),
-                                           I.y_scale_exp, B, C, I.Nr, I.Nr, 2 * I.Nt, alpha, rs,
-                                           ph, bin, cout);
-}
-
-union MergedGemmSmem {
-  fpgemm16::Smem<256, 128, 4, 2> a;
-  fpgemm16::Smem<128, 256, 2, 4> b;
-};
-
-template <int STAGE>
-__global__ __launch_bounds__(512, 1) void batch_gemm_merged(const fp_batch_instance* __restrict__ insts,
-                                                            BatchGroup g) {
-  __shared__ MergedGemmSmem sm;
-  const int total = g.begin[g.n];
-  for (int it = blockIdx.x; it < total; it += gridDim.x) {
-    const int item = (gridDim.x >= total || gridDim.x % 8 == 0) ? fpgemm16::xcd_remap(it, total) : it;
-    const int j = group_find(g, item);
-    const fp_batch_instance& I = insts[g.idx[j]];
-    const int local = item - g.begin[j];
-    if (I.Nr % 256 == 0)
-      gemm_item_merged<256, 128, 4, 2, 1, STAGE>(to_lds(reinterpret_cast<char*>(&sm)), I, local);
-    else
-      gemm_item_merged<128, 256, 2, 4, 0, STAGE>(to_lds(reinterpret_cast<char*>(&sm)), I, local);
-    __syncthreads();
-  }
+  fpgemm16::gemm_tile<BM, BN, 4, 2, 1>(sm, tm, tn, Yh, Yl, yse, B, C, Nr, Nr, 2 * Nt, alpha, rs,
+                                      ph, bin, cout);
 }
 
 }  // namespace
@@ -588,8 +573,7 @@ void rebuild_groups(fp_batch_plan& P) {
       const fp_batch_instance& I = P.host[i];
       rows.push_back(I.Nr);
       groups.push_back(I.Nr / (4096 / I.Nt));
-      tiles.push_back(I.Nr % 256 == 0 ? (I.Nr / 256) * (2 * I.Nt / 128)
-                                      : (I.Nr / 128) * (2 * I.Nt / 256));
+      tiles.push_back((I.Nr / 256) * (2 * I.Nt / 128));   // 256 x 128 tiles
     }
     fp_batch_plan::Launch key{};
     key.ste = any_ste;
@@ -729,34 +713,32 @@ int launch_batch_gemm(const fp_batch_plan& P, const fp_batch_plan::Launch& L, in
 }
 
 int run_pass_merged(fp_batch_plan& P, int pass, hipStream_t st) {
-  const int cus = device_cus();
-  auto grid = [&](const BatchGroup& g, int per_cu) {
+  // one block per work item
+  auto grid = [&](const BatchGroup& g) {
     const int total = g.begin[g.n];
-    int blocks = cus * per_cu;
-    if (blocks > total) blocks = total;
-    return dim3(blocks < 1 ? 1 : blocks);
+    return dim3(total < 1 ? 1 : total);
   };
   switch (pass) {
     case FP_PASS_PLASMA:
       for (const auto& L : P.plasma) {
         if (L.ste)
-          hipLaunchKernelGGL((batch_plasma_merged<true>), grid(L.g, 1), dim3(256), 0, st, P.dev, L.g);
+          hipLaunchKernelGGL((batch_plasma_merged<true>), grid(L.g), dim3(256), 0, st, P.dev, L.g);
         else
-          hipLaunchKernelGGL((batch_plasma_merged<false>), grid(L.g, 2), dim3(256), 0, st, P.dev,
+          hipLaunchKernelGGL((batch_plasma_merged<false>), grid(L.g), dim3(256), 0, st, P.dev,
                              L.g);
       }
       return check_launch("batch_plasma_merged");
     case FP_PASS_NL:
       for (const auto& L : P.nl) {
         if (L.ste)
-          hipLaunchKernelGGL((batch_nl_merged<true>), grid(L.g, 2), dim3(512), 0, st, P.dev, L.g);
+          hipLaunchKernelGGL((batch_nl_merged<true>), grid(L.g), dim3(512), 0, st, P.dev, L.g);
         else
-          hipLaunchKernelGGL((batch_nl_merged<false>), grid(L.g, 2), dim3(512), 0, st, P.dev, L.g);
+          hipLaunchKernelGGL((batch_nl_merged<false>), grid(L.g), dim3(512), 0, st, P.dev, L.g);
       }
       return check_launch("batch_nl_merged");
     case FP_PASS_G1: case FP_PASS_G2: case FP_PASS_G3: case FP_PASS_G4: {
       for (const auto& L : P.gemm) {
-        const dim3 gr = grid(L.g, 1);
+        const dim3 gr = grid(L.g);
         switch (pass) {
           case FP_PASS_G1: hipLaunchKernelGGL((batch_gemm_merged<0>), gr, dim3(512), 0, st, P.dev, L.g); break;
           case FP_PASS_G2: hipLaunchKernelGGL((batch_gemm_merged<1>), gr, dim3(512), 0, st, P.dev, L.g); break;
@@ -1424,8 +1406,8 @@ int fp_batch_set_merged(fp_batch_plan* plan, int merged) {
   if (!plan) return fail(FP_E_ARG, "fp_batch_set_merged: null plan");
   if (merged) {
     for (int i = 0; i < plan->n; ++i)
-      if (plan->host[i].Nr % 128)
-        return fail(FP_E_SHAPE, "fp_batch_set_merged: one launch per pass needs Nr % 128 == 0");
+      if (plan->host[i].Nr % 256)
+        return fail(FP_E_SHAPE, "fp_batch_set_merged: one launch per pass needs Nr % 256 == 0");
   }
   plan->merged = merged != 0;
   plan->dirty = true;
