@@ -268,6 +268,33 @@ FP_DEV int group_find(const BatchGroup& g, int item) {
   return lo;
 }
 
+// A device pointer stored in an instance descriptor, read as a global-memory pointer. A
+// pointer the kernel loads from memory is generic, so the accesses through it compile to
+// flat instructions: 64-bit VGPR addresses, and each flat access also counts against the LDS
+// wait counter, so every wait for an LDS read then waits for the outstanding flat loads too.
+// Read through a global-address-space type, the uses become global instructions (scalar base
+// plus 32-bit offset), as for the single-instance kernels' pointer arguments.
+template <typename T>
+FP_DEV T* gp(T* const& field) {
+  using GT = __attribute__((address_space(1))) T;
+  return (T*)*reinterpret_cast<GT* const*>(reinterpret_cast<const char*>(&field));
+}
+
+// The float32 nonlinear-step tables of a descriptor with every pointer read by gp().
+FP_DEV fp_nl_tables_f32 gtables(const fp_nl_tables_f32& t) {
+  fp_nl_tables_f32 r;
+  r.tw = gp(t.tw);
+  r.mA = gp(t.mA);
+  r.mB = gp(t.mB);
+  r.mC = gp(t.mC);
+  r.Rf = gp(t.Rf);
+  r.knots = gp(t.knots);
+  r.inv_h = gp(t.inv_h);
+  r.pcoef_s = gp(t.pcoef_s);
+  r.rw = gp(t.rw);
+  return r;
+}
+
 // PERS: a persistent grid (blocks loop over the items); otherwise one block per item, with
 // no loop state to carry (the 4096-sample scan then fits the single-instance kernel's
 // register budget: three blocks per CU, no spills).
@@ -283,12 +310,12 @@ __global__ __launch_bounds__(TPB, (PlasmaOcc<float, STE>::value)) void batch_pla
     // follow a store through a pointer could alias the descriptors, so the compiler would
     // make them vector loads, holding each pointer in VGPRs (74 spilled VGPRs at 4096)
     const fp_plasma_params pp = I.pp;
-    const float* spec = I.spec;
-    double *rho = I.rho, *rho_s = STE ? I.rho_s : nullptr;
-    const double *W = I.W_lut, *Ws = I.Ws_lut;
-    const float* tw = I.tw;
-    int* status = I.status;
-    unsigned* slots = I.slots;
+    const float* spec = gp(I.spec);
+    double *rho = gp(I.rho), *rho_s = STE ? gp(I.rho_s) : nullptr;
+    const double *W = gp(I.W_lut), *Ws = gp(I.Ws_lut);
+    const float* tw = gp(I.tw);
+    int* status = gp(I.status);
+    unsigned* slots = gp(I.slots);
     if (row == 0 && threadIdx.x < 3) slots[1 + threadIdx.x] = 0u;
     plasma_row_body<float, NT, TPB, STE, true>(smem, row, spec, rho, rho_s, nullptr, pp, W, Ws,
                                                tw, status);
@@ -312,8 +339,9 @@ __global__ __launch_bounds__(TPB, (TPB >= 512 ? 4 : 2)) void batch_nl_kernel(
     const int j = group_find(g, item);
     const fp_batch_instance& I = insts[g.idx[j]];
     const int row = item - g.begin[j];
-    fpnl32::nl_step_row<NT, TPB, RS>(sm, row, I.w0, I.w0, I.rho, RS ? I.rho_s : nullptr, 1, 1,
-                                     I.np, I.nt, I.slots + 2);
+    float* w0 = gp(I.w0);
+    fpnl32::nl_step_row<NT, TPB, RS>(sm, row, w0, w0, gp(I.rho), RS ? gp(I.rho_s) : nullptr, 1, 1,
+                                     I.np, gtables(I.nt), gp(I.slots) + 2);
   };
   if constexpr (PERS) {
     const int total = g.begin[g.n];
@@ -347,14 +375,15 @@ void batch_gemm_kernel(const fp_batch_instance* __restrict__ insts, BatchGroup g
     const int local = item - g.begin[j];
     const int tilesM = I.Nr / BM;
     //                G1        G2       G3        G4
-    const float* B = STAGE == 0 ? I.spec : STAGE == 2 ? I.w0 : I.w1;
-    float* C = STAGE == 1 ? I.w0 : STAGE == 3 ? I.spec : I.w1;
+    const float* B = gp(STAGE == 0 ? I.spec : STAGE == 2 ? I.w0 : I.w1);
+    float* C = gp(STAGE == 1 ? I.w0 : STAGE == 3 ? I.spec : I.w1);
     const float alpha = (STAGE & 1) ? I.iRk : I.Rk;
-    const float* rs = STAGE == 3 ? I.mask_r : nullptr;
-    const float* ph = (STAGE & 1) ? nullptr : I.phase;
-    const unsigned* bin = I.slots + STAGE;
-    unsigned* cout = STAGE == 0 ? I.slots + 1 : STAGE == 2 ? I.slots + 3 : STAGE == 3 ? I.slots : nullptr;
-    if (STAGE == 1 && local == 0 && threadIdx.x == 0) I.slots[0] = 0u;
+    const float* rs = STAGE == 3 ? gp(I.mask_r) : nullptr;
+    const float* ph = (STAGE & 1) ? nullptr : gp(I.phase);
+    unsigned* slots = gp(I.slots);
+    const unsigned* bin = slots + STAGE;
+    unsigned* cout = STAGE == 0 ? slots + 1 : STAGE == 2 ? slots + 3 : STAGE == 3 ? slots : nullptr;
+    if (STAGE == 1 && local == 0 && threadIdx.x == 0) slots[0] = 0u;
     const int tm = local % tilesM, tn = local / tilesM;
     if (tn * BN >= I.zc_lo && (tn + 1) * BN <= I.zc_hi) {
       // masked frequencies: the product is exactly zero there (phase x mask = 0 in G1 / G3,
@@ -371,8 +400,8 @@ void batch_gemm_kernel(const fp_batch_instance* __restrict__ insts, BatchGroup g
     // phase prefetch of one 16-row block per wave where the registers allow it (the
     // 128 x 128 and 128 x 256 tiles would spill)
     constexpr int NPF = (BM == 128 && BN >= 128) ? 0 : 1;
-    fpgemm16::gemm_tile<BM, BN, WM, WN, NPF>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(I.Yh),
-                                reinterpret_cast<const fpgemm16::h8*>(I.Yl), I.y_scale_exp, B, C,
+    fpgemm16::gemm_tile<BM, BN, WM, WN, NPF>(sm, tm, tn, reinterpret_cast<const fpgemm16::h8*>(gp(I.Yh)),
+                                reinterpret_cast<const fpgemm16::h8*>(gp(I.Yl)), I.y_scale_exp, B, C,
                                 I.Nr, I.Nr, 2 * I.Nt, alpha, rs, ph, bin, cout);
     __syncthreads();
   }
@@ -406,12 +435,12 @@ __global__ __launch_bounds__(256, (PlasmaOcc<float, HAS_STE>::value)) void batch
   const int Nt = I.Nt;
   const bool ste = HAS_STE && I.enable_ste;
   const fp_plasma_params pp = I.pp;
-  const float* spec = I.spec;
-  double *rho = I.rho, *rho_s = HAS_STE ? I.rho_s : nullptr;
-  const double *W = I.W_lut, *Ws = I.Ws_lut;
-  const float* tw = I.tw;
-  int* status = I.status;
-  unsigned* slots = I.slots;
+  const float* spec = gp(I.spec);
+  double *rho = gp(I.rho), *rho_s = HAS_STE ? gp(I.rho_s) : nullptr;
+  const double *W = gp(I.W_lut), *Ws = gp(I.Ws_lut);
+  const float* tw = gp(I.tw);
+  int* status = gp(I.status);
+  unsigned* slots = gp(I.slots);
   if (row == 0 && threadIdx.x < 3) slots[1 + threadIdx.x] = 0u;
 #define FP_MPL(NT)                                                                             \
   if (ste)                                                                                     \
@@ -440,7 +469,7 @@ FP_DEV void nl_rows_merged(char* smem, const fp_batch_instance& I, int grp, floa
   const int sub = threadIdx.x / TPB;
   auto& sm = reinterpret_cast<fpnl32::StepSmem<NT>*>(smem)[sub];
   fpnl32::nl_step_row<NT, TPB, RS>(sm, (size_t)grp * RPB + sub, w0, w0, rho, RS ? rho_s : nullptr,
-                                   1, 1, I.np, I.nt, cmax);
+                                   1, 1, I.np, gtables(I.nt), cmax);
 }
 
 template <bool HAS_STE>
@@ -454,10 +483,10 @@ __global__ __launch_bounds__(512, 4) void batch_nl_merged(const fp_batch_instanc
   const int grp = item - g.begin[j];
   const int Nt = I.Nt;
   const bool ste = HAS_STE && I.enable_ste;
-  float* w0 = I.w0;
-  const double* rho = I.rho;
-  const double* rho_s = HAS_STE ? I.rho_s : nullptr;
-  unsigned* cmax = I.slots + 2;
+  float* w0 = gp(I.w0);
+  const double* rho = gp(I.rho);
+  const double* rho_s = HAS_STE ? gp(I.rho_s) : nullptr;
+  unsigned* cmax = gp(I.slots) + 2;
 #define FP_MNL(NT)                                                      \
   if (ste) nl_rows_merged<NT, HAS_STE>(smem, I, grp, w0, rho, rho_s, cmax); \
   else nl_rows_merged<NT, false>(smem, I, grp, w0, rho, rho_s, cmax);       \
@@ -486,15 +515,15 @@ __global__ __launch_bounds__(512, 1) void batch_gemm_merged(const fp_batch_insta
   const fp_batch_instance& I = insts[__builtin_amdgcn_readfirstlane(g.idx[j])];
   const int local = item - g.begin[j];
   const int Nr = I.Nr, Nt = I.Nt, zc_lo = I.zc_lo, zc_hi = I.zc_hi, yse = I.y_scale_exp;
-  const auto* Yh = reinterpret_cast<const fpgemm16::h8*>(I.Yh);
-  const auto* Yl = reinterpret_cast<const fpgemm16::h8*>(I.Yl);
+  const auto* Yh = reinterpret_cast<const fpgemm16::h8*>(gp(I.Yh));
+  const auto* Yl = reinterpret_cast<const fpgemm16::h8*>(gp(I.Yl));
   //                G1        G2       G3        G4
-  const float* B = STAGE == 0 ? I.spec : STAGE == 2 ? I.w0 : I.w1;
-  float* C = STAGE == 1 ? I.w0 : STAGE == 3 ? I.spec : I.w1;
+  const float* B = gp(STAGE == 0 ? I.spec : STAGE == 2 ? I.w0 : I.w1);
+  float* C = gp(STAGE == 1 ? I.w0 : STAGE == 3 ? I.spec : I.w1);
   const float alpha = (STAGE & 1) ? I.iRk : I.Rk;
-  const float* rs = STAGE == 3 ? I.mask_r : nullptr;
-  const float* ph = (STAGE & 1) ? nullptr : I.phase;
-  unsigned* slots = I.slots;
+  const float* rs = STAGE == 3 ? gp(I.mask_r) : nullptr;
+  const float* ph = (STAGE & 1) ? nullptr : gp(I.phase);
+  unsigned* slots = gp(I.slots);
   const unsigned* bin = slots + STAGE;
   unsigned* cout = STAGE == 0 ? slots + 1 : STAGE == 2 ? slots + 3 : STAGE == 3 ? slots : nullptr;
   if (STAGE == 1 && local == 0 && threadIdx.x == 0) slots[0] = 0u;

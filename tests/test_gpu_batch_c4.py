@@ -90,6 +90,9 @@ def test_sweep_classes_tuned_graph_vs_single(case):
     _announce(f"sweep classes {[(c.N - 1, c.Nt) for c in cfgs]}: tuned + graph, {STEPS} steps")
     b = BatchedIntegrator(cfgs, envs, tune=True)
     _announce(f"  tuned: {dict((k, v) for k, v in b.config.items() if k != 'timings_ms')}")
+    # the tuned per-class shapes in grouped launches, whichever launch mode the tuner chose
+    # (one launch per pass has its own test below)
+    b.set_merged(False)
     g = b.capture(1)
     for _ in range(STEPS):
         g.replay()
