@@ -461,6 +461,55 @@ def run_cpu_baseline(n_steps=5, repeats=3):
                        f"1024x4096 workload, NumPy/SciPy FP64 oracle, BLAS threads = {_cores()}")
 
 
+def run_small_throughput(repeats=3, cpu_steps=20, gpu=True, cpu=True):
+    """The CPU-sized case (128 x 512, 200 steps of 10 um, recording off) as a whole run:
+    z-steps/s of the FP64 and mixed GPU paths (the 200 steps as graph replays from the same
+    input, after one warm-up run; median of `repeats` runs) and of the NumPy FP64 oracle on the
+    host (median of `repeats` runs of `cpu_steps` steps after one warm-up step)."""
+    import statistics
+    out = dict(shape="128 x 512", steps=200, recording="off")
+    if gpu:
+        import torch
+        import rapport_amd as R
+        n = SMALL["nz"] + 1
+        for prec in ("fp64", "mixed"):
+            kw = dict(SMALL, save_stride=10 ** 9, precision=prec)
+            it = R.Integrator(R.Config(**kw), envelope=R.envelope_lens_noise(**SMALL_ENVELOPE))
+            g = it.capture_step(1)
+            snap = it._snapshot()
+            rates = []
+            for r in range(repeats + 1):
+                it._restore(snap)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(n):
+                    g.replay()
+                torch.cuda.synchronize()
+                if r > 0:
+                    rates.append(n / (time.perf_counter() - t0))
+            out[prec] = dict(value=statistics.median(rates), unit="z-steps/s", runs=rates)
+            del g, it
+        out["mixed_speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if cpu:
+        from oracle import reference_numpy as O
+        kw = dict(SMALL, save_stride=10 ** 9)
+        it = O.Integrator(O.Config(**kw), envelope=O.envelope_lens_noise(**SMALL_ENVELOPE))
+        it.advance(1)
+        rates, i = [], 2
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            for _ in range(cpu_steps):
+                it.advance(i)
+                i += 1
+            rates.append(cpu_steps / (time.perf_counter() - t0))
+        out["cpu_baseline"] = dict(value=statistics.median(rates), unit="z-steps/s",
+                                   cores=_cores(), kind="port", runs=rates,
+                                   sample=f"median of {repeats} runs of {cpu_steps} z-steps "
+                                          "(after 1 warm-up) of the 128x512 run, NumPy/SciPy "
+                                          f"FP64 oracle, BLAS threads = {_cores()}")
+    return out
+
+
 def copy_calibration(nbytes=256 * 2 ** 20, reps=5):
     """A plain device-to-device copy of a known size: the FETCH_SIZE / WRITE_SIZE
     counters of its kernel calibrate the counters read for the propagator kernels."""
@@ -630,6 +679,8 @@ def main():
         out["batched"] = run_batched(args.batched, args.batched_steps)
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint(fit_iters=args.fit_iters)
+    if args.precision == "both" and not args.no_small_accuracy:
+        out["small_128x512"] = run_small_throughput(cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)
