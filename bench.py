@@ -613,7 +613,7 @@ def main():
     ap.add_argument("--sources-steps", type=int, default=4000,
                     help="steps of the error-source runs (floor and stage swaps; 0: skip)")
     ap.add_argument("--no-small-accuracy", action="store_true",
-                    help="skip the mixed-vs-FP64 check at 128 x 512")
+                    help="skip the 128 x 512 case (mixed-vs-FP64 check and whole-run z-steps/s)")
     ap.add_argument("--batched", type=int, default=64,
                     help="instances of the batched sweep (0: skip the batched measurement)")
     ap.add_argument("--batched-steps", type=int, default=20)
