@@ -407,6 +407,13 @@ int fp_batch_set_grid(fp_batch_plan* plan, int kind, int blocks);
  * group of instances sharing a kernel configuration (the default). */
 int fp_batch_set_merged(fp_batch_plan* plan, int merged);
 
+/* Launch mode per pass: bit FP_PASS_x of mask set = that pass is one launch over the merged
+ * work list of every instance (as fp_batch_set_merged(plan, 1) does for all six), clear = the
+ * grouped launches. fp_batch_set_merged(plan, m) is this call with mask 0x3F (m != 0) or 0.
+ * Needs Nr % 256 == 0 when mask != 0 (FP_E_SHAPE otherwise, the plan keeps its mode).
+ * No counterpart in the reference (the batched form is new). */
+int fp_batch_set_merged_passes(fp_batch_plan* plan, int mask);
+
 /* One pass (FP_PASS_*) for every instance, or the whole z-step (fp_batch_step_mixed). */
 int fp_batch_pass(fp_batch_plan* plan, int pass, void* stream);
 int fp_batch_step_mixed(fp_batch_plan* plan, void* stream);

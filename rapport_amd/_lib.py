@@ -107,6 +107,7 @@ _SIGS = {
     "fp_batch_set_config": [P, c_int, c_int, c_int, c_int],
     "fp_batch_set_grid": [P, c_int, c_int],
     "fp_batch_set_merged": [P, c_int],
+    "fp_batch_set_merged_passes": [P, c_int],
     "fp_batch_pass": [P, c_int, P],
     "fp_batch_step_mixed": [P, P],
     "fp_batch_launches": [P],

@@ -10,8 +10,8 @@ events on the launch stream (median of ``reps`` after ``warmup`` launches):
   nonlinear pass, which works in place on the scratch buffer w0;
 * grid of each pass kind over the whole batch: one block per work item or a
   persistent grid of k x 256 blocks;
-* launch mode: those grouped launches, or one launch per pass over the merged work
-  lists of every instance (fixed shapes), whichever passes take less time.
+* launch mode per pass: those grouped launches, or one launch over the merged work
+  list of every instance (fixed shapes), whichever takes less time.
 
 Only the scratch buffers and the scale words are touched; the spectral state is
 not. Before timing, w1 and w0 are loaded with copies of the state (so the GEMM
@@ -126,25 +126,29 @@ def tune(batch, warmup=2, reps=5, grids=True, verbose=False):
             res["grid"][kind] = best[1]
             batch.set_grid(kind, best[1])
 
-    # Launch mode: the tuned grouped launches, or one launch per pass over merged work lists
-    # (fixed shapes; needs Nr % 256 == 0), by the summed time of the passes that write only
-    # scratch buffers, scale words and the carrier densities: the scan, the nonlinear step and
-    # four times G2 (G1 / G3 / G4 are the same product, with a phase or a row scale). Fewer
-    # launches win a tie within 1 %.
-    def passes_ms():
-        return (_time(lambda: call("fp_batch_pass", batch._plan, 0, st), warmup, reps)
-                + 4 * _time(lambda: call("fp_batch_pass", batch._plan, 2, st), warmup, reps)
-                + _time(lambda: call("fp_batch_pass", batch._plan, 3, st), warmup, reps))
-
-    t_grouped = passes_ms()
-    res["timings_ms"]["mode grouped"] = t_grouped
-    res["merged"] = False
+    # Launch mode per pass: the tuned grouped launches, or one launch over the merged work
+    # list of every instance (fixed shapes; needs Nr % 256 == 0). Timed are the passes that
+    # write only scratch buffers, scale words and the carrier densities: the scan, G1, G2, the
+    # nonlinear step and G3; G4 (which writes the state) follows G2, the other product without
+    # a phase. One launch wins a tie within 1 %.
+    res["merged_passes"] = 0
     if all(m.Nr % 256 == 0 for m in ms):
-        batch.set_merged(True)
-        t_merged = passes_ms()
-        res["timings_ms"]["mode merged"] = t_merged
-        res["merged"] = t_merged <= 1.01 * t_grouped
-        batch.set_merged(res["merged"])
+        timed = (0, 1, 2, 3, 4)
+        t_g = {p: _time(lambda: call("fp_batch_pass", batch._plan, p, st), warmup, reps)
+               for p in timed}
+        batch.set_merged_passes(0x3F)
+        t_m = {p: _time(lambda: call("fp_batch_pass", batch._plan, p, st), warmup, reps)
+               for p in timed}
+        mask = 0
+        for p in timed:
+            res["timings_ms"][f"pass {p} grouped"] = t_g[p]
+            res["timings_ms"][f"pass {p} merged"] = t_m[p]
+            if t_m[p] <= 1.01 * t_g[p]:
+                mask |= 1 << p
+        if mask & (1 << 2):
+            mask |= 1 << 5
+        res["merged_passes"] = mask
+        batch.set_merged_passes(mask)
     batch.refresh_scales()
     torch.cuda.synchronize()
     batch.config = res
@@ -161,5 +165,5 @@ def apply(batch, res):
         batch.set_config(i, bm, bn, tpb)
     for kind, g in res.get("grid", {}).items():
         batch.set_grid(kind, g)
-    batch.set_merged(bool(res.get("merged", False)))
+    batch.set_merged_passes(int(res.get("merged_passes", 0)))
     batch.config = res

@@ -114,7 +114,14 @@ class BatchedIntegrator:
         """One launch per pass for the whole batch (six launches per z-step) instead of
         one per group of instances sharing a kernel configuration."""
         call("fp_batch_set_merged", self._plan, int(bool(merged)))
-        self.merged = bool(merged)
+        self.merged = 0x3F if merged else 0
+        self._graph = None
+
+    def set_merged_passes(self, mask):
+        """Launch mode per pass: bit p (pass p: 0 scan, 1 G1, 2 G2, 3 nonlinear step, 4 G3,
+        5 G4) set = one launch for the whole batch, clear = grouped launches."""
+        call("fp_batch_set_merged_passes", self._plan, int(mask))
+        self.merged = int(mask)
         self._graph = None
 
     def launches_per_step(self):

@@ -558,9 +558,10 @@ struct fp_batch_plan {
     int Nt, ste, a, b;          // row kernels: a = threads per block; GEMM: a x b tile
     BatchGroup g;
   };
-  std::vector<Launch> plasma, nl, gemm;
+  std::vector<Launch> plasma, nl, gemm;           // grouped launches
+  std::vector<Launch> plasma_m, nl_m, gemm_m;     // merged work lists (one launch per pass)
   bool dirty = true;
-  bool merged = false;        // one launch per pass for the whole batch
+  unsigned merged = 0;        // bit FP_PASS_x: that pass is one launch for the whole batch
 };
 
 namespace {
@@ -586,6 +587,9 @@ void rebuild_groups(fp_batch_plan& P) {
   P.plasma.clear();
   P.nl.clear();
   P.gemm.clear();
+  P.plasma_m.clear();
+  P.nl_m.clear();
+  P.gemm_m.clear();
   if (P.merged) {
     // every instance in one group per pass, heavier instances first
     std::vector<int> order(P.n);
@@ -606,11 +610,9 @@ void rebuild_groups(fp_batch_plan& P) {
     }
     fp_batch_plan::Launch key{};
     key.ste = any_ste;
-    push_group(P.plasma, key, order, rows);
-    push_group(P.nl, key, order, groups);
-    push_group(P.gemm, key, order, tiles);
-    P.dirty = false;
-    return;
+    push_group(P.plasma_m, key, order, rows);
+    push_group(P.nl_m, key, order, groups);
+    push_group(P.gemm_m, key, order, tiles);
   }
   std::vector<std::array<int, 4>> keys;
   for (int kind = 0; kind < 3; ++kind) {
@@ -749,7 +751,7 @@ int run_pass_merged(fp_batch_plan& P, int pass, hipStream_t st) {
   };
   switch (pass) {
     case FP_PASS_PLASMA:
-      for (const auto& L : P.plasma) {
+      for (const auto& L : P.plasma_m) {
         if (L.ste)
           hipLaunchKernelGGL((batch_plasma_merged<true>), grid(L.g), dim3(256), 0, st, P.dev, L.g);
         else
@@ -758,7 +760,7 @@ int run_pass_merged(fp_batch_plan& P, int pass, hipStream_t st) {
       }
       return check_launch("batch_plasma_merged");
     case FP_PASS_NL:
-      for (const auto& L : P.nl) {
+      for (const auto& L : P.nl_m) {
         if (L.ste)
           hipLaunchKernelGGL((batch_nl_merged<true>), grid(L.g), dim3(512), 0, st, P.dev, L.g);
         else
@@ -766,7 +768,7 @@ int run_pass_merged(fp_batch_plan& P, int pass, hipStream_t st) {
       }
       return check_launch("batch_nl_merged");
     case FP_PASS_G1: case FP_PASS_G2: case FP_PASS_G3: case FP_PASS_G4: {
-      for (const auto& L : P.gemm) {
+      for (const auto& L : P.gemm_m) {
         const dim3 gr = grid(L.g);
         switch (pass) {
           case FP_PASS_G1: hipLaunchKernelGGL((batch_gemm_merged<0>), gr, dim3(512), 0, st, P.dev, L.g); break;
@@ -783,7 +785,7 @@ int run_pass_merged(fp_batch_plan& P, int pass, hipStream_t st) {
 
 int run_pass(fp_batch_plan& P, int pass, hipStream_t st) {
   if (P.dirty) rebuild_groups(P);
-  if (P.merged) return run_pass_merged(P, pass, st);
+  if (pass >= 0 && pass < 6 && ((P.merged >> pass) & 1u)) return run_pass_merged(P, pass, st);
   int rc = FP_OK;
   switch (pass) {
     case FP_PASS_PLASMA:
@@ -1432,13 +1434,19 @@ int fp_batch_set_grid(fp_batch_plan* plan, int kind, int blocks) {
 }
 
 int fp_batch_set_merged(fp_batch_plan* plan, int merged) {
-  if (!plan) return fail(FP_E_ARG, "fp_batch_set_merged: null plan");
+  return fp_batch_set_merged_passes(plan, merged ? 0x3F : 0);
+}
+
+int fp_batch_set_merged_passes(fp_batch_plan* plan, int mask) {
+  if (!plan) return fail(FP_E_ARG, "fp_batch_set_merged_passes: null plan");
+  if (mask & ~0x3F) return fail(FP_E_ARG, "fp_batch_set_merged_passes: bits beyond the six passes");
+  const int merged = mask;
   if (merged) {
     for (int i = 0; i < plan->n; ++i)
       if (plan->host[i].Nr % 256)
         return fail(FP_E_SHAPE, "fp_batch_set_merged: one launch per pass needs Nr % 256 == 0");
   }
-  plan->merged = merged != 0;
+  plan->merged = (unsigned)merged;
   plan->dirty = true;
   return FP_OK;
 }
@@ -1462,7 +1470,13 @@ int fp_batch_step_mixed(fp_batch_plan* plan, void* stream) {
 int fp_batch_launches(const fp_batch_plan* plan) {
   if (!plan) return -1;
   if (plan->dirty) rebuild_groups(const_cast<fp_batch_plan&>(*plan));
-  return (int)(plan->plasma.size() + plan->nl.size() + 4 * plan->gemm.size());
+  const fp_batch_plan& P = *plan;
+  auto on = [&](int pass) { return ((P.merged >> pass) & 1u) != 0; };
+  int n = (int)(on(FP_PASS_PLASMA) ? P.plasma_m.size() : P.plasma.size()) +
+          (int)(on(FP_PASS_NL) ? P.nl_m.size() : P.nl.size());
+  for (int pass : {FP_PASS_G1, FP_PASS_G2, FP_PASS_G3, FP_PASS_G4})
+    n += (int)(on(pass) ? P.gemm_m.size() : P.gemm.size());
+  return n;
 }
 
 

@@ -155,3 +155,19 @@ def test_sweep_classes_merged_single_launch_per_pass_vs_single(case):
         g.replay()
     _check(b, singles, "merged")
     b.close()
+
+
+def test_sweep_classes_mixed_launch_modes_vs_single(case):
+    """G1 and G3 as one launch each over the merged work list, the other passes grouped."""
+    from rapport_amd.batch import BatchedIntegrator
+    cfgs, envs, singles = case
+    _announce("sweep classes: G1 / G3 merged, other passes grouped, graph, 3 steps")
+    b = BatchedIntegrator(cfgs, envs)
+    grouped = b.launches_per_step()
+    b.set_merged_passes((1 << 1) | (1 << 4))
+    assert 6 < b.launches_per_step() < grouped
+    g = b.capture(1)
+    for _ in range(STEPS):
+        g.replay()
+    _check(b, singles, "G1 / G3 merged")
+    b.close()
